@@ -1,0 +1,137 @@
+/*
+ * omdg — C ABI of the MI355X (gfx950) force engine for Object-MD.
+ *
+ * One context owns the device copy of one system: FP64 positions and
+ * velocities, forces (FP32 with compensated accumulation, or FP64), the
+ * neighbour list and the potential tables.  All calls on a context must come
+ * from one host thread; every call returns OMDG_OK (0) or an error code, and
+ * omdg_last_error() gives the message of the last failure on that thread.
+ *
+ * Atom arrays crossing the boundary are in the caller's atom order (the
+ * reference's AtomStorage order); the engine permutes atoms into cell order
+ * internally and undoes the permutation on download.
+ */
+#ifndef OMDG_H
+#define OMDG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct omdg_ctx* omdg_t;
+
+enum {
+  OMDG_OK = 0,
+  OMDG_EINVAL = 1,       /* bad argument or call order */
+  OMDG_EHIP = 2,         /* HIP runtime failure */
+  OMDG_ERANGE_HI = 3,    /* spline read above the table (treader.cpp:199-206) */
+  OMDG_ERANGE_LO = 4,    /* spline read below the table (treader.cpp:208-218) */
+  OMDG_EOVERLAP = 5,     /* r^2 < 1e-6 (gadget.cpp:149-159) */
+  OMDG_ENBR = 6,         /* neighbour list overflow or skin exceeded */
+  OMDG_ENODEV = 7        /* no HIP device */
+};
+
+/* download fields */
+enum { OMDG_X = 0, OMDG_V = 1, OMDG_F = 2, OMDG_POT = 3, OMDG_VIR = 4,
+       OMDG_RHO = 5 };
+/* table kinds: the three tables of an eam.<name> file (team.cpp:47-66,
+ * 206-222) and the single table of a pair.<name> file (tpair.h:55-60) */
+enum { OMDG_TAB_PAIR = 0, OMDG_TAB_EDENS = 1, OMDG_TAB_EMBED = 2 };
+/* potentials */
+enum { OMDG_POT_NONE = 0, OMDG_POT_PAIR = 1, OMDG_POT_EAM = 2,
+       OMDG_POT_SW = 3, OMDG_POT_TERSOFF = 4 };
+/* atom flags (base.h:230-232), plus a fixed-atom bit for clamped layers */
+enum { OMDG_FLAG_ACTIVE = 1, OMDG_FLAG_FIXED = 8 };
+
+/* Context lifetime.  Replaces the device-less MDSystem/MDIntegrator pair
+ * (system.h:139-410, integrator.h:41-69) as the owner of atom state. */
+int omdg_create(omdg_t* out, int hip_device);
+int omdg_destroy(omdg_t ctx);
+const char* omdg_last_error(void);
+
+/* Simulation box and periodicity.  lo/hi as SysBox x0..z1 (system.h,
+ * Crystalline.h:53-74); pbc_mask bit 0/1/2 = periodic x/y/z
+ * (boundary.periodic, systemgrid.cpp:93-152). */
+int omdg_set_box(omdg_t ctx, const double lo[3], const double hi[3],
+                 int pbc_mask);
+
+/* Atom state, AoS triples in caller order.  Replaces the Atom records
+ * (base.h:114-125) that the integrator and force kernels touch: x, v, tid,
+ * flag; mass per type in eV ps^2/A^2 (container.cpp:94).  flag may be NULL
+ * (all active). */
+int omdg_upload_atoms(omdg_t ctx, int n, const double* x3, const double* v3,
+                      const int* tid, const int* flag, int ntypes,
+                      const double* mass_by_type);
+/* Overwrite positions and/or velocities (NULL keeps the current values). */
+int omdg_set_state(omdg_t ctx, const double* x3, const double* v3);
+/* Copy a field back in caller order: X/V/F 3n doubles, POT/VIR/RHO n. */
+int omdg_download(omdg_t ctx, int field, double* out);
+
+/* Spline tables.  Raw equispaced samples y[0..n) with Spacing dr, Offset r0,
+ * Format RMULT when rmult != 0; the coefficients are computed as
+ * TableReader::calculate_coef does (treader.cpp:37-64).  ta/tb select the
+ * atom-type pair (EMBED uses ta only). */
+int omdg_add_table(omdg_t ctx, int kind, int ta, int tb, int n, double dr,
+                   double r0, int rmult, const double* y);
+/* Same, reading the "#$<name>" block of a table file (treader.cpp:83-136). */
+int omdg_load_table(omdg_t ctx, int kind, int ta, int tb, const char* path,
+                    const char* name);
+
+/* Select the potential.  OMDG_POT_PAIR replaces TForcePair::ComputeHalf
+ * (tpair.h:62-71); OMDG_POT_EAM replaces TEmbedding (team.cpp:81-145) and
+ * TForceEAM::ComputeHalf (team.cpp:224-237); OMDG_POT_SW replaces
+ * StillingerWeber (sw.cpp:118-273); OMDG_POT_TERSOFF replaces the Tersoff
+ * bond order of tersoff.h:259-365.  fp64 = 1 computes forces in FP64
+ * (plain method), 0 in FP32 with compensated per-atom sums. */
+int omdg_set_potential(omdg_t ctx, int potential, int fp64);
+
+/* Stillinger-Weber parameters in the order of tables/sw.silicon and
+ * sw.cpp:35-60: eps A B p q sigma a lambda gamma invers_cos0 usezbl
+ * sx0 sx1 sa0 sa1 sa2 sa3 Z1 Z2 (19 values). */
+int omdg_set_sw(omdg_t ctx, const double* p19);
+/* Tersoff parameters, keys of tables/tersoff.silicon (tersoff.h:146-180):
+ * A B lambda_1 lambda_2 lambda_3 alpha beta n c d h R D (13 values);
+ * cutoff function on [R-D, R+D] (tersoff.h:247-257). */
+int omdg_set_tersoff(omdg_t ctx, const double* p13);
+
+/* Verlet list control.  Replaces VerletList/VerletListFull parameters
+ * verlet.radtole / verlet.n_mean (VerletList.cpp:46-52) and the refresh
+ * cadence comm.refresh (systemgrid.cpp:565-617): the list radius is
+ * cut + skin, displacements are checked every check_every steps and the
+ * list is rebuilt before any atom moves more than skin/2.  maxnbr = 0
+ * sizes the list from the first build. */
+int omdg_set_neighbor(omdg_t ctx, double skin, int check_every, int maxnbr);
+/* Build now from the current positions (VerletList::Update,
+ * VerletList.cpp:126-199). */
+int omdg_neighbor_build(omdg_t ctx);
+/* Export the full list in caller order: cnt[i] and, per atom, the sorted
+ * codes j*27 + img with img = (sx+1) + 3(sy+1) + 9(sz+1) the periodic image
+ * of j.  Half list = entries with j > i (or j == i, img > 13), as
+ * VerletList::Update enumerates them; full list = VerletListFull. */
+int omdg_neighbor_export(omdg_t ctx, int maxn, int* cnt, int* code);
+
+/* One force evaluation on the current positions = MDIntegrator::Iterate
+ * (integrator.cpp:203-216) with the PRE_CALCULATION/POST_FORCE work of
+ * TEmbedding.  Returns the potential energy sum and virial/2
+ * (MDSystem::MeasurePotential, system.cpp:1060-1075); either may be NULL. */
+int omdg_compute(omdg_t ctx, double* epot, double* virial);
+/* Zero the forces, as MDSystem::FirstRun does after its evaluation
+ * (system.cpp:872). */
+int omdg_zero_forces(omdg_t ctx);
+/* nsteps of MDIntegrator::Integrate (integrator.cpp:225-259) with time step
+ * dt (ps).  After the last step ekin = MeasureKinetic (system.cpp:1077-1094)
+ * and epot = MeasurePotential; either may be NULL. */
+int omdg_step(omdg_t ctx, int nsteps, double dt, double* ekin, double* epot);
+
+/* Kernel timing (no counterpart in the reference, whose only timer has
+ * one-second resolution, system.cpp:635): with enable = 1 every launch
+ * group is bracketed by HIP events on the context's stream.  A call with
+ * enable = 0 stops recording and returns, per group, the summed milliseconds
+ * and the launch count (groups: 0 list build, 1 density/first pass, 2 force
+ * pass, 3 kick+drift, 4 kick, 5 reductions).  max = size of the arrays. */
+int omdg_timing(omdg_t ctx, int enable, int max, double* ms, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

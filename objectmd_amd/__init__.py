@@ -1,0 +1,216 @@
+"""objectmd_amd — MI355X force engine for Object-MD.
+
+Python access to the C ABI of ``include/omdg.h`` (``libomdg.so``, built in
+tree by ``python -m objectmd_amd.build``).  The HIP library is the only
+compute path: if it is missing or no GPU is present, :class:`Engine` raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libomdg.so")
+
+OK, EINVAL, EHIP, ERANGE_HI, ERANGE_LO, EOVERLAP, ENBR, ENODEV = range(8)
+X, V, F, POT, VIR, RHO = range(6)
+TAB_PAIR, TAB_EDENS, TAB_EMBED = range(3)
+POT_PAIR, POT_EAM, POT_SW, POT_TERSOFF = 1, 2, 3, 4
+FLAG_ACTIVE, FLAG_FIXED = 1, 8
+
+# every entry point declared in include/omdg.h
+ABI_SYMBOLS = (
+    "omdg_create", "omdg_destroy", "omdg_last_error", "omdg_set_box",
+    "omdg_upload_atoms", "omdg_set_state", "omdg_download", "omdg_add_table",
+    "omdg_load_table", "omdg_set_potential", "omdg_set_sw", "omdg_set_tersoff",
+    "omdg_set_neighbor", "omdg_neighbor_build", "omdg_neighbor_export",
+    "omdg_compute", "omdg_zero_forces", "omdg_step", "omdg_timing",
+)
+TIMING_GROUPS = ("build", "pass1", "force", "kick_drift", "kick", "reduce")
+
+_lib = None
+
+
+class OmdgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"omdg error {code}: {msg}")
+        self.code = code
+
+
+def load_library(path=LIB_PATH):
+    """Load libomdg.so and declare the C signatures (no GPU call)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OmdgError(EINVAL, f"{path} is not built: run python -m objectmd_amd.build")
+    lib = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "omdg_create": [ctypes.POINTER(P), ctypes.c_int],
+        "omdg_destroy": [P],
+        "omdg_set_box": [P, dp, dp, ctypes.c_int],
+        "omdg_upload_atoms": [P, ctypes.c_int, dp, dp, ip, ip, ctypes.c_int, dp],
+        "omdg_set_state": [P, dp, dp],
+        "omdg_download": [P, ctypes.c_int, dp],
+        "omdg_add_table": [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                           ctypes.c_double, ctypes.c_double, ctypes.c_int, dp],
+        "omdg_load_table": [P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                            ctypes.c_char_p, ctypes.c_char_p],
+        "omdg_set_potential": [P, ctypes.c_int, ctypes.c_int],
+        "omdg_set_sw": [P, dp],
+        "omdg_set_tersoff": [P, dp],
+        "omdg_set_neighbor": [P, ctypes.c_double, ctypes.c_int, ctypes.c_int],
+        "omdg_neighbor_build": [P],
+        "omdg_neighbor_export": [P, ctypes.c_int, ip, ip],
+        "omdg_compute": [P, dp, dp],
+        "omdg_zero_forces": [P],
+        "omdg_step": [P, ctypes.c_int, ctypes.c_double, dp, dp],
+        "omdg_timing": [P, ctypes.c_int, ctypes.c_int, dp, ip],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    lib.omdg_last_error.argtypes = []
+    lib.omdg_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class Engine:
+    """One device context (omdg_t) holding one system."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        self.h = ctypes.c_void_p()
+        self._check(self.lib.omdg_create(ctypes.byref(self.h), device))
+        self.n = 0
+
+    def _check(self, rc):
+        if rc != OK:
+            msg = self.lib.omdg_last_error().decode(errors="replace")
+            raise OmdgError(rc, msg)
+
+    def close(self):
+        if self.h:
+            self.lib.omdg_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- setup -----------------------------------------------------------------
+    def set_box(self, lo, hi, pbc):
+        lo = np.ascontiguousarray(lo, dtype=np.float64)
+        hi = np.ascontiguousarray(hi, dtype=np.float64)
+        self._check(self.lib.omdg_set_box(self.h, _dptr(lo), _dptr(hi), int(pbc)))
+
+    def upload_atoms(self, x, v=None, tid=None, flag=None, mass=(1.0,)):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        n = x.shape[0]
+        v = (np.zeros_like(x) if v is None
+             else np.ascontiguousarray(v, dtype=np.float64).reshape(n, 3))
+        tid = (np.zeros(n, np.int32) if tid is None
+               else np.ascontiguousarray(tid, dtype=np.int32))
+        mass = np.ascontiguousarray(mass, dtype=np.float64)
+        fl = None if flag is None else np.ascontiguousarray(flag, dtype=np.int32)
+        self._check(self.lib.omdg_upload_atoms(
+            self.h, n, _dptr(x), _dptr(v), _iptr(tid),
+            _iptr(fl) if fl is not None else None, len(mass), _dptr(mass)))
+        self.n = n
+
+    def set_state(self, x=None, v=None):
+        xa = None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+        va = None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+        self._check(self.lib.omdg_set_state(
+            self.h, _dptr(xa) if xa is not None else None,
+            _dptr(va) if va is not None else None))
+
+    def add_table(self, kind, ta, tb, y, dr, r0, rmult=False):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        self._check(self.lib.omdg_add_table(self.h, kind, ta, tb, len(y), float(dr),
+                                            float(r0), int(bool(rmult)), _dptr(y)))
+
+    def load_table(self, kind, ta, tb, path, name):
+        self._check(self.lib.omdg_load_table(self.h, kind, ta, tb, path.encode(),
+                                             name.encode()))
+
+    def set_potential(self, pot, fp64=False):
+        self._check(self.lib.omdg_set_potential(self.h, pot, int(bool(fp64))))
+
+    def set_sw(self, p19):
+        p = np.ascontiguousarray(p19, dtype=np.float64)
+        assert p.size == 19
+        self._check(self.lib.omdg_set_sw(self.h, _dptr(p)))
+
+    def set_tersoff(self, p13):
+        p = np.ascontiguousarray(p13, dtype=np.float64)
+        assert p.size == 13
+        self._check(self.lib.omdg_set_tersoff(self.h, _dptr(p)))
+
+    def set_neighbor(self, skin, check_every=10, maxnbr=0):
+        self._check(self.lib.omdg_set_neighbor(self.h, float(skin), int(check_every),
+                                               int(maxnbr)))
+
+    # -- work ------------------------------------------------------------------
+    def neighbor_build(self):
+        self._check(self.lib.omdg_neighbor_build(self.h))
+
+    def neighbor_export(self, maxn=256):
+        cnt = np.zeros(self.n, np.int32)
+        code = np.zeros((self.n, maxn), np.int32)
+        self._check(self.lib.omdg_neighbor_export(self.h, maxn, _iptr(cnt), _iptr(code)))
+        return cnt, code
+
+    def compute(self):
+        ep = ctypes.c_double()
+        vir = ctypes.c_double()
+        self._check(self.lib.omdg_compute(self.h, ctypes.byref(ep), ctypes.byref(vir)))
+        return ep.value, vir.value
+
+    def zero_forces(self):
+        self._check(self.lib.omdg_zero_forces(self.h))
+
+    def step(self, nsteps, dt):
+        ek = ctypes.c_double()
+        ep = ctypes.c_double()
+        self._check(self.lib.omdg_step(self.h, int(nsteps), float(dt), ctypes.byref(ek),
+                                       ctypes.byref(ep)))
+        return ek.value, ep.value
+
+    def timing_start(self):
+        self._check(self.lib.omdg_timing(self.h, 1, 0, None, None))
+
+    def timing_stop(self):
+        """-> {group: (total ms, launches)} of the launches since timing_start."""
+        k = len(TIMING_GROUPS)
+        ms = np.zeros(k)
+        cnt = np.zeros(k, np.int32)
+        self._check(self.lib.omdg_timing(self.h, 0, k, _dptr(ms), _iptr(cnt)))
+        return {g: (float(ms[i]), int(cnt[i])) for i, g in enumerate(TIMING_GROUPS)}
+
+    def download(self, field):
+        shape = (self.n, 3) if field in (X, V, F) else (self.n,)
+        out = np.empty(shape, np.float64)
+        self._check(self.lib.omdg_download(self.h, field, _dptr(out)))
+        return out

@@ -1,0 +1,72 @@
+"""Build the in-tree shared libraries.
+
+libomdg.so   the HIP engine for gfx950 (objectmd_amd/csrc/*.hip) behind the
+             C ABI of include/omdg.h
+libomd.so    the C++ Object-MD host API over libomdg (objectmd_amd/host)
+
+Run as ``python -m objectmd_amd.build``; __graft_entry__.build() calls
+:func:`build_all`.
+"""
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_SOURCES = ["omdg_api.hip", "nbr.hip", "eam.hip", "pair.hip",
+               "threebody.hip", "integrate.hip"]
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result",
+]
+
+
+def lib_path(name="libomdg.so"):
+    return os.path.join(PKG, name)
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_engine():
+    out = lib_path("libomdg.so")
+    tmp = out + ".tmp"
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), *srcs,
+          "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_host_api():
+    """C++ mirror of the Object-MD plugin API (objectmd_amd/host)."""
+    host = os.path.join(PKG, "host")
+    if not os.path.isdir(host):
+        return None
+    srcs = sorted(os.path.join(host, "src", f)
+                  for f in os.listdir(os.path.join(host, "src"))
+                  if f.endswith(".cpp"))
+    out = lib_path("libomd.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+          "-I", os.path.join(host, "include"), "-I", os.path.join(ROOT, "include"),
+          *srcs, "-o", tmp, "-L", PKG, "-lomdg", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all():
+    build_engine()
+    build_host_api()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

@@ -1,0 +1,387 @@
+// Neighbour-list construction on the device (replaces VerletList::Update,
+// src/gadget/VerletList.cpp:126-199, and VerletListFull::Update,
+// src/gadget/VerletListFull.cpp:138-230).
+//
+// Pipeline (one rebuild):
+//   wrap_aabb  wrap periodic coordinates into [lo, lo+L) with the reference's
+//              image arithmetic (x + L / x + (-L)); block min/max of the free
+//              dimensions
+//   bin        cell coordinates of every atom, per-cell counts
+//   scan       exclusive prefix sum of the counts (one workgroup)
+//   scatter    counting-sort slots; cells then sorted by original index so the
+//              order is deterministic
+//   permute    gather every per-atom array into cell order (double buffer)
+//   build      one thread per atom scans the 27 stencil cells, tests
+//              r^2 <= rlist^2 in FP64 exactly as the reference (shift the
+//              image first, subtract, sum without FMA) and writes the ELL list
+//              nbr[k * n + i] = slot | stencil-offset << 27; also writes the
+//              FP32 cell-relative snapshot used by the FP32 force kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "omdg_internal.h"
+
+namespace omdg {
+
+namespace {
+
+__global__ void k_wrap_aabb(double* __restrict__ x, int n, double3 lo, double3 L, int pbc,
+                            double* __restrict__ part, int* __restrict__ stat) {
+  __shared__ double smin[3][256];
+  __shared__ double smax[3][256];
+  const int tid = threadIdx.x;
+  double mn[3] = {1e308, 1e308, 1e308}, mx[3] = {-1e308, -1e308, -1e308};
+  const double lod[3] = {lo.x, lo.y, lo.z};
+  const double Ld[3] = {L.x, L.y, L.z};
+  for (int i = blockIdx.x * blockDim.x + tid; i < n; i += gridDim.x * blockDim.x) {
+    for (int d = 0; d < 3; d++) {
+      double v = x[(size_t)d * n + i];
+      if ((pbc >> d) & 1) {
+        // re-ownership of an atom that left the box: the image inside
+        // becomes the atom (systemgrid.cpp:479-506, comhandler.cpp:288-355)
+        int guard = 0;
+        while (v < lod[d] && guard < 2) { v = v + Ld[d]; guard++; }
+        while (v >= lod[d] + Ld[d] && guard < 2) { v = v + (-Ld[d]); guard++; }
+        if (v < lod[d] || v >= lod[d] + Ld[d]) atomicOr(stat + ST_ERR, ERR_WRAP);
+        x[(size_t)d * n + i] = v;
+      }
+      mn[d] = fmin(mn[d], v);
+      mx[d] = fmax(mx[d], v);
+    }
+  }
+  for (int d = 0; d < 3; d++) {
+    smin[d][tid] = mn[d];
+    smax[d][tid] = mx[d];
+  }
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (tid < s)
+      for (int d = 0; d < 3; d++) {
+        smin[d][tid] = fmin(smin[d][tid], smin[d][tid + s]);
+        smax[d][tid] = fmax(smax[d][tid], smax[d][tid + s]);
+      }
+    __syncthreads();
+  }
+  if (tid == 0)
+    for (int d = 0; d < 3; d++) {
+      part[blockIdx.x * 6 + d] = smin[d][0];
+      part[blockIdx.x * 6 + 3 + d] = smax[d][0];
+    }
+}
+
+__global__ void k_aabb_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double r[6] = {1e308, 1e308, 1e308, -1e308, -1e308, -1e308};
+  for (int b = 0; b < nb; b++)
+    for (int d = 0; d < 3; d++) {
+      r[d] = fmin(r[d], part[b * 6 + d]);
+      r[3 + d] = fmax(r[3 + d], part[b * 6 + 3 + d]);
+    }
+  for (int k = 0; k < 6; k++) out[k] = r[k];
+}
+
+struct GridDev {
+  int M[3];
+  double origin[3];
+  double inv_h[3];
+  double h[3];
+  double L[3];
+  int pbc;
+};
+
+__device__ inline int cell_coord(double v, double origin, double inv_h, int M) {
+  int k = (int)floor((v - origin) * inv_h);
+  return k < 0 ? 0 : (k > M - 1 ? M - 1 : k);
+}
+
+__global__ void k_bin(const double* __restrict__ x, int n, GridDev g, int* __restrict__ cellid,
+                      int* __restrict__ count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int k[3];
+  for (int d = 0; d < 3; d++)
+    k[d] = cell_coord(x[(size_t)d * n + i], g.origin[d], g.inv_h[d], g.M[d]);
+  int c = k[0] + g.M[0] * (k[1] + g.M[1] * k[2]);
+  cellid[i] = c;
+  atomicAdd(count + c, 1);
+}
+
+// exclusive scan of count[0..nc) into start[0..nc], count reset to zero
+__global__ void __launch_bounds__(1024) k_scan(int* __restrict__ count, int* __restrict__ start,
+                                               int nc) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (nc + 1023) / 1024;
+  const int b = t * per, e = min(nc, b + per);
+  int s = 0;
+  for (int c = b; c < e; c++) s += count[c];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+  for (int c = b; c < e; c++) {
+    int v = count[c];
+    start[c] = run;
+    run += v;
+    count[c] = 0;
+  }
+  if (t == 1023) start[nc] = part[1023];
+}
+
+__global__ void k_scatter(const int* __restrict__ cellid, int n, const int* __restrict__ start,
+                          int* __restrict__ fill, int* __restrict__ perm) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = cellid[i];
+  int slot = start[c] + atomicAdd(fill + c, 1);
+  perm[slot] = i;
+}
+
+// order each cell by original atom index (insertion sort; cells are small)
+__global__ void k_cell_sort(int* __restrict__ perm, const int* __restrict__ start, int nc,
+                            const int* __restrict__ id) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  int b = start[c], e = start[c + 1];
+  for (int i = b + 1; i < e; i++) {
+    int p = perm[i];
+    int key = id[p];
+    int j = i - 1;
+    while (j >= b && id[perm[j]] > key) {
+      perm[j + 1] = perm[j];
+      j--;
+    }
+    perm[j + 1] = p;
+  }
+}
+
+__global__ void k_permute(int n, const int* __restrict__ perm, const double* __restrict__ x0,
+                          const double* __restrict__ v0, const int* __restrict__ t0,
+                          const int* __restrict__ f0, const int* __restrict__ id0,
+                          const int* __restrict__ c0, double* __restrict__ x1,
+                          double* __restrict__ v1, int* __restrict__ t1, int* __restrict__ f1,
+                          int* __restrict__ id1, int* __restrict__ c1) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int p = perm[s];
+  for (int d = 0; d < 3; d++) {
+    x1[(size_t)d * n + s] = x0[(size_t)d * n + p];
+    v1[(size_t)d * n + s] = v0[(size_t)d * n + p];
+  }
+  t1[s] = t0[p];
+  f1[s] = f0[p];
+  id1[s] = id0[p];
+  c1[s] = c0[p];
+}
+
+struct BuildArgs {
+  const double* x;
+  const int* type;
+  const int* flag;
+  const int* cellid;
+  const int* start;
+  uint32_t* nbr;
+  int* nnbr;
+  float4* q;
+  float4* qb;
+  double* xb;
+  int* stat;
+  int n;
+  int maxnbr;
+  int has_flags;
+  double rl2;
+  GridDev g;
+};
+
+__device__ inline double shifted(double x, int s, double L) {
+  return s == 0 ? x : (s < 0 ? x + (-L) : x + L);
+}
+
+__global__ void __launch_bounds__(256) k_build(BuildArgs a) {
+#pragma clang fp contract(off)
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  const int n = a.n;
+  const double xi = a.x[s], yi = a.x[(size_t)n + s], zi = a.x[2 * (size_t)n + s];
+  const int c = a.cellid[s];
+  const int k0 = c % a.g.M[0], k1 = (c / a.g.M[0]) % a.g.M[1], k2 = c / (a.g.M[0] * a.g.M[1]);
+  // FP32 snapshot relative to the cell origin
+  const double ox = a.g.origin[0] + (double)k0 * a.g.h[0];
+  const double oy = a.g.origin[1] + (double)k1 * a.g.h[1];
+  const double oz = a.g.origin[2] + (double)k2 * a.g.h[2];
+  float4 qv;
+  qv.x = (float)(xi - ox);
+  qv.y = (float)(yi - oy);
+  qv.z = (float)(zi - oz);
+  qv.w = __int_as_float(a.type[s]);
+  a.q[s] = qv;
+  a.qb[s] = qv;
+  a.xb[s] = xi;
+  a.xb[(size_t)n + s] = yi;
+  a.xb[2 * (size_t)n + s] = zi;
+  if (a.has_flags && !(a.flag[s] & OMDG_FLAG_ACTIVE)) {
+    a.nnbr[s] = 0;
+    return;
+  }
+  int cnt = 0;
+  const int pbc = a.g.pbc;
+  for (int dz = -1; dz <= 1; dz++) {
+    int kz = k2 + dz, sz = 0;
+    if ((pbc >> 2) & 1) {
+      if (kz < 0) { kz += a.g.M[2]; sz = -1; }
+      else if (kz >= a.g.M[2]) { kz -= a.g.M[2]; sz = 1; }
+    } else if (kz < 0 || kz >= a.g.M[2]) {
+      continue;
+    }
+    for (int dy = -1; dy <= 1; dy++) {
+      int ky = k1 + dy, sy = 0;
+      if ((pbc >> 1) & 1) {
+        if (ky < 0) { ky += a.g.M[1]; sy = -1; }
+        else if (ky >= a.g.M[1]) { ky -= a.g.M[1]; sy = 1; }
+      } else if (ky < 0 || ky >= a.g.M[1]) {
+        continue;
+      }
+      for (int dx = -1; dx <= 1; dx++) {
+        int kx = k0 + dx, sx = 0;
+        if (pbc & 1) {
+          if (kx < 0) { kx += a.g.M[0]; sx = -1; }
+          else if (kx >= a.g.M[0]) { kx -= a.g.M[0]; sx = 1; }
+        } else if (kx < 0 || kx >= a.g.M[0]) {
+          continue;
+        }
+        const int cc = kx + a.g.M[0] * (ky + a.g.M[1] * kz);
+        const uint32_t img = (uint32_t)((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1));
+        const bool noshift = (sx | sy | sz) == 0;
+        const int e = a.start[cc + 1];
+        for (int t = a.start[cc]; t < e; t++) {
+          if (t == s && noshift) continue;
+          if (a.has_flags && !(a.flag[t] & OMDG_FLAG_ACTIVE)) continue;
+          const double ddx = xi - shifted(a.x[t], sx, a.g.L[0]);
+          const double ddy = yi - shifted(a.x[(size_t)n + t], sy, a.g.L[1]);
+          const double ddz = zi - shifted(a.x[2 * (size_t)n + t], sz, a.g.L[2]);
+          const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+          if (r2 <= a.rl2) {
+            if (cnt < a.maxnbr) a.nbr[(size_t)cnt * n + s] = (uint32_t)t | (img << kImgShift);
+            cnt++;
+          }
+        }
+      }
+    }
+  }
+  // the stored count never exceeds the storage; an overflow is reported and
+  // the host rebuilds with a longer list before any force kernel runs
+  a.nnbr[s] = cnt < a.maxnbr ? cnt : a.maxnbr;
+  if (cnt > a.maxnbr) atomicOr(a.stat + ST_ERR, ERR_NBR_OVERFLOW);
+  atomicMax(a.stat + ST_MAXNBR, cnt);
+}
+
+__global__ void k_reset_build_stats(int* stat) {
+  if (threadIdx.x == 0) {
+    stat[ST_DISP2] = 0;
+    stat[ST_VMAX2] = 0;
+    stat[ST_MAXNBR] = 0;
+  }
+}
+
+}  // namespace
+
+void nbr_build(Ctx& c) {
+  const int n = c.n;
+  const int B = 256;
+  const int nb = std::min(1024, grid_blocks(n, B));
+  double* part = c.dred;  // 8 * red_blocks >= 6 * nb doubles
+  // 1. wrap + bounding box
+  k_wrap_aabb<<<nb, B, 0, c.stream>>>(c.x[c.cur], n, make_double3(c.lo[0], c.lo[1], c.lo[2]),
+                                      make_double3(c.L[0], c.L[1], c.L[2]), c.pbc, part, c.dstat);
+  k_aabb_final<<<1, 64, 0, c.stream>>>(part, nb, c.aabb);
+  double bb[6];
+  OMDG_HIP(hipMemcpyAsync(bb, c.aabb, sizeof bb, hipMemcpyDeviceToHost, c.stream));
+  OMDG_HIP(hipStreamSynchronize(c.stream));
+  // 2. grid: cells no smaller than rlist (VerletList.cpp:115-117); periodic
+  //    dimensions tile the box exactly so the stencil wraps consistently
+  Grid& g = c.grid;
+  g.pbc = c.pbc;
+  const double rl = c.rlist;
+  for (int d = 0; d < 3; d++) {
+    double ext, org;
+    if ((c.pbc >> d) & 1) {
+      ext = c.L[d];
+      org = c.lo[d];
+      if (ext < rl)
+        throw ApiFail(OMDG_EINVAL, "periodic box shorter than the list radius");
+    } else {
+      ext = bb[3 + d] - bb[d];
+      org = bb[d];
+    }
+    int M = (int)(ext / rl);
+    if (M < 1) M = 1;
+    if (M > 1023) M = 1023;
+    g.M[d] = M;
+    g.origin[d] = org;
+    g.h[d] = ext > 0 ? ext / M : 1.0;
+    g.L[d] = c.L[d];
+  }
+  const long nc = g.ncell();
+  if (nc + 1 > c.cap_cells) {
+    (void)hipFree(c.cell_start);
+    (void)hipFree(c.cell_fill);
+    c.cell_start = c.cell_fill = nullptr;
+    OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.cell_start), sizeof(int) * (nc + 1)));
+    OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.cell_fill), sizeof(int) * (nc + 1)));
+    c.cap_cells = nc + 1;
+  }
+  if ((long)c.maxnbr * n > c.cap_nbr) throw ApiFail(OMDG_EINVAL, "list storage too small");
+  GridDev gd;
+  for (int d = 0; d < 3; d++) {
+    gd.M[d] = g.M[d];
+    gd.origin[d] = g.origin[d];
+    gd.h[d] = g.h[d];
+    gd.inv_h[d] = 1.0 / g.h[d];
+    gd.L[d] = g.L[d];
+  }
+  gd.pbc = g.pbc;
+  OMDG_HIP(hipMemsetAsync(c.cell_fill, 0, sizeof(int) * (nc + 1), c.stream));
+  const int src = c.cur, dst = 1 - c.cur;
+  k_bin<<<grid_blocks(n, B), B, 0, c.stream>>>(c.x[src], n, gd, c.cellid, c.cell_fill);
+  k_scan<<<1, 1024, 0, c.stream>>>(c.cell_fill, c.cell_start, (int)nc);
+  k_scatter<<<grid_blocks(n, B), B, 0, c.stream>>>(c.cellid, n, c.cell_start, c.cell_fill,
+                                                   c.perm);
+  k_cell_sort<<<grid_blocks(nc, B), B, 0, c.stream>>>(c.perm, c.cell_start, (int)nc, c.id[src]);
+  int* cell_dst = c.cellid2;
+  k_permute<<<grid_blocks(n, B), B, 0, c.stream>>>(
+      n, c.perm, c.x[src], c.v[src], c.type[src], c.flag[src], c.id[src], c.cellid, c.x[dst],
+      c.v[dst], c.type[dst], c.flag[dst], c.id[dst], cell_dst);
+  OMDG_HIP(hipMemcpyAsync(c.cellid, cell_dst, sizeof(int) * n, hipMemcpyDeviceToDevice,
+                          c.stream));
+  c.cur = dst;
+  k_reset_build_stats<<<1, 64, 0, c.stream>>>(c.dstat);
+  BuildArgs a;
+  a.x = c.x[c.cur];
+  a.type = c.type[c.cur];
+  a.flag = c.flag[c.cur];
+  a.cellid = c.cellid;
+  a.start = c.cell_start;
+  a.nbr = c.nbr;
+  a.nnbr = c.nnbr;
+  a.q = c.q;
+  a.qb = c.qb;
+  a.xb = c.xb;
+  a.stat = c.dstat;
+  a.n = n;
+  a.maxnbr = c.maxnbr;
+  a.has_flags = c.has_flags ? 1 : 0;
+  a.rl2 = rl * rl;
+  a.g = gd;
+  k_build<<<grid_blocks(n, 256), 256, 0, c.stream>>>(a);
+  OMDG_HIP(hipGetLastError());
+  c.forces_valid = false;
+}
+
+}  // namespace omdg

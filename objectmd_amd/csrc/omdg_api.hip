@@ -1,0 +1,869 @@
+// C ABI of the MI355X force engine: context, atom state, tables, the step
+// loop.  Kernels live in nbr.hip, eam.hip, pair.hip, threebody.hip and
+// integrate.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "omdg_internal.h"
+
+namespace omdg {
+
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+
+int grid_blocks(long n, int block) {
+  long b = (n + block - 1) / block;
+  return (int)std::max(1L, b);
+}
+
+// ---------------------------------------------------------------------------
+// spline coefficients: parabolic run-out cubic spline of TableReader
+// (treader.cpp:37-64).  S_i = h^2 M_i / 6, S_{i-1} + 4 S_i + S_{i+1} equals
+// the second difference of the samples, S_0 = S_1 and S_{n-1} = S_{n-2}.
+// ---------------------------------------------------------------------------
+static void spline_coefficients(int n, double dr, const double* y,
+                                std::vector<double>& abcd) {
+  abcd.assign(4 * (size_t)n, 0.0);
+  for (int i = 0; i < n; i++) abcd[4 * i + 3] = y[i];
+  const int m = n - 2;
+  if (m < 1) return;
+  std::vector<double> diag(m, 4.0), rhs(m), S(n);
+  diag[0] = 5.0;
+  diag[m - 1] = 5.0;
+  for (int i = 0; i < m; i++) rhs[i] = y[i + 2] - 2.0 * y[i + 1] + y[i];
+  for (int k = 1; k < m; k++) {
+    const double w = 1.0 / diag[k - 1];
+    diag[k] -= w;
+    rhs[k] -= w * rhs[k - 1];
+  }
+  S[m] = rhs[m - 1] / diag[m - 1];
+  S[m + 1] = S[m];
+  for (int k = m - 2; k >= 0; k--) S[k + 1] = (rhs[k] - S[k + 2]) / diag[k];
+  S[0] = S[1];
+  for (int i = 0; i + 1 < n; i++) {
+    abcd[4 * i + 0] = (S[i + 1] - S[i]) / dr / dr / dr;
+    abcd[4 * i + 1] = 3.0 * S[i] / dr / dr;
+    abcd[4 * i + 2] = (y[i + 1] - y[i] - 2.0 * S[i] - S[i + 1]) / dr;
+  }
+}
+
+// "#$<name> key value" header lines, samples after "#$<name> --"
+// (param.cpp:106-139, treader.cpp:83-136).
+static void read_table_file(const char* path, const char* name,
+                            std::vector<double>& y, double& dr, double& r0,
+                            int& rmult) {
+  std::ifstream fl(path);
+  if (!fl.good()) throw ApiFail(OMDG_EINVAL, std::string("cannot open table ") + path);
+  const std::string mark = std::string("#$") + name;
+  std::string line;
+  int n = -1;
+  bool have_dr = false, found = false;
+  r0 = 0.0;
+  rmult = 0;
+  while (std::getline(fl, line)) {
+    std::istringstream ss(line);
+    std::string tok, key, val;
+    if (!(ss >> tok) || tok != mark) continue;
+    if (!(ss >> key)) continue;
+    if (key == "--") {
+      found = true;
+      break;
+    }
+    if (!(ss >> val)) continue;
+    if (key == "NumberOfData") n = std::stoi(val);
+    else if (key == "Spacing") { dr = std::stod(val); have_dr = true; }
+    else if (key == "Offset") r0 = std::stod(val);
+    else if (key == "Format") {
+      if (val == "RMULT") rmult = 1;
+      else if (val == "PLAIN") rmult = 0;
+      else throw ApiFail(OMDG_EINVAL, "unsupported table format " + val);
+    }
+  }
+  if (!found || n < 3 || !have_dr)
+    throw ApiFail(OMDG_EINVAL, std::string("can not find table ") + name + "@" + path);
+  y.resize(n);
+  for (int i = 0; i < n; i++)
+    if (!(fl >> y[i])) throw ApiFail(OMDG_EINVAL, std::string("short table ") + name + "@" + path);
+}
+
+static hipEvent_t next_event(Ctx& c) {
+  if (c.ev_used == c.ev_pool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c.ev_pool.push_back(e);
+  }
+  return c.ev_pool[c.ev_used++];
+}
+
+TimerScope::TimerScope(Ctx& ctx, int g) : c(ctx), group(g), on(ctx.timing) {
+  if (!on) return;
+  hipEvent_t e = next_event(c);
+  if (!e || hipEventRecord(e, c.stream) != hipSuccess) on = false;
+}
+
+TimerScope::~TimerScope() {
+  if (!on) return;
+  hipEvent_t e = next_event(c);
+  if (e && hipEventRecord(e, c.stream) == hipSuccess) c.ev_group.push_back(group);
+  else c.timing = false;
+}
+
+TabView tab_view(const Ctx& c, int tid) {
+  TabView v{};
+  if (tid < 0) return v;
+  const DevTable& t = c.tables[tid];
+  v.n = t.n;
+  v.rmult = t.rmult;
+  v.r0 = t.r0_f;
+  v.dr = t.dr_f;
+  v.inv_dr = t.inv_dr_f;
+  v.r0d = t.r0;
+  v.drd = t.dr;
+  v.c32 = reinterpret_cast<const float4*>(c.tab32) + t.off;
+  v.c64 = reinterpret_cast<const double4*>(c.tab64) + t.off;
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+template <class T>
+static void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+template <class T>
+static void dalloc(T*& p, size_t count) {
+  dfree(p);
+  if (count == 0) count = 1;
+  OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count));
+}
+
+static void free_atoms(Ctx& c) {
+  for (int b = 0; b < 2; b++) {
+    dfree(c.x[b]);
+    dfree(c.v[b]);
+    dfree(c.type[b]);
+    dfree(c.flag[b]);
+    dfree(c.id[b]);
+  }
+  dfree(c.f64);
+  dfree(c.f32);
+  dfree(c.q);
+  dfree(c.qb);
+  dfree(c.xb);
+  dfree(c.cellid);
+  dfree(c.cellid2);
+  dfree(c.nnbr);
+  dfree(c.pot32);
+  dfree(c.vir32);
+  dfree(c.pot64);
+  dfree(c.vir64);
+  dfree(c.rho);
+  dfree(c.fp);
+  dfree(c.emb);
+  dfree(c.perm);
+  dfree(c.dred);
+  c.cap_atoms = 0;
+}
+
+static void alloc_atoms(Ctx& c, int n) {
+  free_atoms(c);
+  for (int b = 0; b < 2; b++) {
+    dalloc(c.x[b], 3 * (size_t)n);
+    dalloc(c.v[b], 3 * (size_t)n);
+    dalloc(c.type[b], n);
+    dalloc(c.flag[b], n);
+    dalloc(c.id[b], n);
+  }
+  dalloc(c.f64, 3 * (size_t)n);
+  dalloc(c.f32, 3 * (size_t)n);
+  dalloc(c.q, n);
+  dalloc(c.qb, n);
+  dalloc(c.xb, 3 * (size_t)n);
+  dalloc(c.cellid, n);
+  dalloc(c.cellid2, n);
+  dalloc(c.nnbr, n);
+  dalloc(c.pot32, n);
+  dalloc(c.vir32, n);
+  dalloc(c.pot64, n);
+  dalloc(c.vir64, n);
+  dalloc(c.rho, n);
+  dalloc(c.fp, n);
+  dalloc(c.emb, n);
+  dalloc(c.perm, n);
+  c.red_blocks = std::min(1024, grid_blocks(n, 256));
+  dalloc(c.dred, 8 * (size_t)c.red_blocks);
+  c.cap_atoms = n;
+}
+
+static void upload_tables(Ctx& c) {
+  if (!c.tables_dirty) return;
+  size_t knots = c.host_c64.size() / 4;
+  std::vector<float> t32(c.host_c64.size());
+  for (size_t i = 0; i < c.host_c64.size(); i++) t32[i] = (float)c.host_c64[i];
+  dalloc(c.tab32, std::max<size_t>(4, 4 * knots));
+  dalloc(c.tab64, std::max<size_t>(4, 4 * knots));
+  if (knots) {
+    OMDG_HIP(hipMemcpy(c.tab32, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice));
+    OMDG_HIP(hipMemcpy(c.tab64, c.host_c64.data(), sizeof(double) * c.host_c64.size(),
+                       hipMemcpyHostToDevice));
+  }
+  c.tables_dirty = false;
+}
+
+static void check_ready(Ctx& c) {
+  if (!c.box_set) throw ApiFail(OMDG_EINVAL, "box not set (omdg_set_box)");
+  if (c.n <= 0) throw ApiFail(OMDG_EINVAL, "no atoms uploaded");
+  if (c.potential == OMDG_POT_NONE) throw ApiFail(OMDG_EINVAL, "no potential selected");
+  upload_tables(c);
+}
+
+static void read_status(Ctx& c) {
+  OMDG_HIP(hipMemcpyAsync(c.hstat, c.dstat, sizeof(int) * ST_COUNT, hipMemcpyDeviceToHost,
+                          c.stream));
+  OMDG_HIP(hipStreamSynchronize(c.stream));
+}
+
+static float bits_to_float(int b) {
+  float f;
+  std::memcpy(&f, &b, sizeof f);
+  return f;
+}
+
+static void raise_device_errors(Ctx& c) {
+  int e = c.hstat[ST_ERR];
+  if (!e) return;
+  int z = 0;
+  OMDG_HIP(hipMemcpy(c.dstat + ST_ERR, &z, sizeof(int), hipMemcpyHostToDevice));
+  c.forces_valid = false;
+  if (e & ERR_OVERLAP) throw ApiFail(OMDG_EOVERLAP, "overlapping atoms: r^2 < 1e-6");
+  if (e & ERR_RANGE_HI) throw ApiFail(OMDG_ERANGE_HI, "high limit exceeded in a spline table");
+  if (e & ERR_RANGE_LO) throw ApiFail(OMDG_ERANGE_LO, "low limit exceeded in a spline table");
+  if (e & ERR_NBR_OVERFLOW)
+    throw ApiFail(OMDG_ENBR, "neighbour list overflow (maxnbr too small)");
+  if (e & ERR_SKIN)
+    throw ApiFail(OMDG_ENBR, "an atom moved more than skin/2 between list checks");
+  if (e & ERR_WRAP) throw ApiFail(OMDG_ENBR, "an atom left the periodic box by more than one length");
+  throw ApiFail(OMDG_EINVAL, "device error flags " + std::to_string(e));
+}
+
+// (Re)build the neighbour list; sizes the list on the first build.
+static void rebuild(Ctx& c) {
+  c.rlist = c.cut + c.skin;
+  if (c.maxnbr == 0) {
+    c.maxnbr = c.maxnbr_req > 0 ? c.maxnbr_req : 64;
+  }
+  for (int attempt = 0; attempt < 3; attempt++) {
+    size_t need = (size_t)c.maxnbr * (size_t)c.n;
+    if ((long)need > c.cap_nbr) {
+      dalloc(c.nbr, need);
+      c.cap_nbr = (long)need;
+    }
+    {
+      TimerScope ts(c, TG_BUILD);
+      nbr_build(c);
+    }
+    read_status(c);
+    int got = c.hstat[ST_MAXNBR];
+    if (!(c.hstat[ST_ERR] & ERR_NBR_OVERFLOW)) {
+      raise_device_errors(c);
+      c.list_valid = true;
+      return;
+    }
+    if (c.maxnbr_req > 0) break;  // caller fixed the size: report overflow
+    int z = 0;
+    OMDG_HIP(hipMemcpy(c.dstat + ST_ERR, &z, sizeof(int), hipMemcpyHostToDevice));
+    c.maxnbr = ((got + got / 4 + 15) / 16) * 16;  // 25% head room
+  }
+  raise_device_errors(c);
+}
+
+static void ensure_list(Ctx& c) {
+  if (!c.list_valid) rebuild(c);
+}
+
+static void compute_forces(Ctx& c, bool energy) {
+  switch (c.potential) {
+    case OMDG_POT_EAM: eam_forces(c, energy); break;  // times its two passes
+    case OMDG_POT_PAIR: {
+      TimerScope ts(c, TG_FORCE);
+      pair_forces(c, energy);
+      break;
+    }
+    case OMDG_POT_SW: sw_forces(c, energy); break;
+    case OMDG_POT_TERSOFF: tersoff_forces(c, energy); break;
+    default: throw ApiFail(OMDG_EINVAL, "no potential selected");
+  }
+  c.forces_valid = true;
+}
+
+static void update_cut(Ctx& c) {
+  double cut = 0;
+  auto tmax = [&](int tid) {
+    const DevTable& t = c.tables[tid];
+    return t.r0 + (double)(t.n - 1) * t.dr;
+  };
+  if (c.potential == OMDG_POT_PAIR || c.potential == OMDG_POT_EAM) {
+    for (int a = 0; a < kMaxTypes; a++)
+      for (int b = 0; b < kMaxTypes; b++) {
+        int tp = c.tab_index[OMDG_TAB_PAIR][a][b];
+        if (tp >= 0) cut = std::max(cut, tmax(tp));
+        if (c.potential == OMDG_POT_EAM) {
+          int te = c.tab_index[OMDG_TAB_EDENS][a][b];
+          if (te >= 0) cut = std::max(cut, tmax(te));
+        }
+      }
+  } else if (c.potential == OMDG_POT_SW) {
+    cut = (double)c.sw.a * (double)c.sw.sigma;
+  } else if (c.potential == OMDG_POT_TERSOFF) {
+    cut = (double)c.ters.R + (double)c.ters.D;
+  }
+  if (cut != c.cut) {
+    c.cut = cut;
+    c.list_valid = false;
+  }
+}
+
+}  // namespace omdg
+
+using namespace omdg;
+
+struct omdg_ctx {
+  Ctx c;
+};
+
+template <class F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return OMDG_OK;
+  } catch (const ApiFail& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const HipFail& e) {
+    set_error(e.msg);
+    return OMDG_EHIP;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return OMDG_EINVAL;
+  }
+}
+
+extern "C" {
+
+const char* omdg_last_error(void) { return g_err.c_str(); }
+
+int omdg_create(omdg_t* out, int hip_device) {
+  if (!out) {
+    set_error("null output pointer");
+    return OMDG_EINVAL;
+  }
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_error("no HIP device available");
+    return OMDG_ENODEV;
+  }
+  if (hip_device < 0 || hip_device >= ndev) {
+    set_error("bad HIP device index");
+    return OMDG_EINVAL;
+  }
+  omdg_ctx* h = new omdg_ctx();
+  int rc = guarded([&] {
+    Ctx& c = h->c;
+    c.device = hip_device;
+    OMDG_HIP(hipSetDevice(hip_device));
+    OMDG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.dstat), sizeof(int) * ST_COUNT));
+    OMDG_HIP(hipMemset(c.dstat, 0, sizeof(int) * ST_COUNT));
+    OMDG_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.hstat), sizeof(int) * ST_COUNT, 0));
+    OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.dres), sizeof(double) * 8));
+    OMDG_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.hres), sizeof(double) * 8, 0));
+    OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.aabb), sizeof(double) * 8));
+    for (auto& k : c.tab_index)
+      for (auto& a : k)
+        for (int& b : a) b = -1;
+  });
+  if (rc != OMDG_OK) {
+    omdg_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return OMDG_OK;
+}
+
+int omdg_destroy(omdg_t h) {
+  if (!h) return OMDG_OK;
+  Ctx& c = h->c;
+  (void)hipSetDevice(c.device);
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  free_atoms(c);
+  dfree(c.nbr);
+  dfree(c.cell_start);
+  dfree(c.cell_fill);
+  dfree(c.tab32);
+  dfree(c.tab64);
+  dfree(c.dstat);
+  dfree(c.dres);
+  dfree(c.aabb);
+  for (hipEvent_t e : c.ev_pool) (void)hipEventDestroy(e);
+  if (c.hstat) (void)hipHostFree(c.hstat);
+  if (c.hres) (void)hipHostFree(c.hres);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+  delete h;
+  return OMDG_OK;
+}
+
+int omdg_set_box(omdg_t h, const double lo[3], const double hi[3], int pbc) {
+  if (!h || !lo || !hi) {
+    set_error("null argument");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    for (int d = 0; d < 3; d++) {
+      if (!(hi[d] > lo[d]) && ((pbc >> d) & 1))
+        throw ApiFail(OMDG_EINVAL, "periodic box needs hi > lo");
+      c.lo[d] = lo[d];
+      c.hi[d] = hi[d];
+      // SysBox::lx = fabs(x1 - x0) (Crystalline.h:64-66)
+      c.L[d] = std::fabs(hi[d] - lo[d]);
+    }
+    c.pbc = pbc & 7;
+    c.box_set = true;
+    c.list_valid = false;
+  });
+}
+
+int omdg_upload_atoms(omdg_t h, int n, const double* x3, const double* v3, const int* tid,
+                      const int* flag, int ntypes, const double* mass) {
+  if (!h || n <= 0 || !x3 || !mass || ntypes < 1 || ntypes > kMaxTypes ||
+      n >= (int)kSlotMask) {
+    set_error("bad argument to omdg_upload_atoms");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    OMDG_HIP(hipSetDevice(c.device));
+    if (c.cap_atoms < n || c.n != n) alloc_atoms(c, n);
+    c.n = n;
+    c.ntypes = ntypes;
+    for (int t = 0; t < ntypes; t++) {
+      if (!(mass[t] > 0)) throw ApiFail(OMDG_EINVAL, "mass must be positive");
+      c.mass[t] = mass[t];
+    }
+    std::vector<double> soa(3 * (size_t)n);
+    for (int d = 0; d < 3; d++)
+      for (int i = 0; i < n; i++) soa[(size_t)d * n + i] = x3[3 * (size_t)i + d];
+    c.cur = 0;
+    OMDG_HIP(hipMemcpy(c.x[0], soa.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    for (int d = 0; d < 3; d++)
+      for (int i = 0; i < n; i++) soa[(size_t)d * n + i] = v3 ? v3[3 * (size_t)i + d] : 0.0;
+    OMDG_HIP(hipMemcpy(c.v[0], soa.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    std::vector<int> iv(n);
+    for (int i = 0; i < n; i++) {
+      int t = tid ? tid[i] : 0;
+      if (t < 0 || t >= ntypes) throw ApiFail(OMDG_EINVAL, "atom type out of range");
+      iv[i] = t;
+    }
+    OMDG_HIP(hipMemcpy(c.type[0], iv.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    c.has_flags = false;
+    for (int i = 0; i < n; i++) {
+      iv[i] = flag ? flag[i] : OMDG_FLAG_ACTIVE;
+      if (iv[i] != OMDG_FLAG_ACTIVE) c.has_flags = true;
+    }
+    OMDG_HIP(hipMemcpy(c.flag[0], iv.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    for (int i = 0; i < n; i++) iv[i] = i;
+    OMDG_HIP(hipMemcpy(c.id[0], iv.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    OMDG_HIP(hipMemset(c.f64, 0, sizeof(double) * 3 * n));
+    OMDG_HIP(hipMemset(c.f32, 0, sizeof(float) * 3 * n));
+    OMDG_HIP(hipMemset(c.dstat, 0, sizeof(int) * ST_COUNT));
+    c.list_valid = false;
+    c.forces_valid = false;
+  });
+}
+
+int omdg_set_state(omdg_t h, const double* x3, const double* v3) {
+  if (!h) {
+    set_error("null context");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    if (c.n <= 0) throw ApiFail(OMDG_EINVAL, "no atoms uploaded");
+    const int n = c.n;
+    std::vector<int> id(n);
+    OMDG_HIP(hipMemcpy(id.data(), c.id[c.cur], sizeof(int) * n, hipMemcpyDeviceToHost));
+    std::vector<double> soa(3 * (size_t)n);
+    if (x3) {
+      for (int s = 0; s < n; s++)
+        for (int d = 0; d < 3; d++) soa[(size_t)d * n + s] = x3[3 * (size_t)id[s] + d];
+      OMDG_HIP(hipMemcpy(c.x[c.cur], soa.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+      c.list_valid = false;
+      c.forces_valid = false;
+    }
+    if (v3) {
+      for (int s = 0; s < n; s++)
+        for (int d = 0; d < 3; d++) soa[(size_t)d * n + s] = v3[3 * (size_t)id[s] + d];
+      OMDG_HIP(hipMemcpy(c.v[c.cur], soa.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    }
+  });
+}
+
+int omdg_download(omdg_t h, int field, double* out) {
+  if (!h || !out) {
+    set_error("null argument");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    const int n = c.n;
+    if (n <= 0) throw ApiFail(OMDG_EINVAL, "no atoms uploaded");
+    OMDG_HIP(hipStreamSynchronize(c.stream));
+    std::vector<int> id(n);
+    OMDG_HIP(hipMemcpy(id.data(), c.id[c.cur], sizeof(int) * n, hipMemcpyDeviceToHost));
+    auto scatter3 = [&](const std::vector<double>& soa) {
+      for (int s = 0; s < n; s++)
+        for (int d = 0; d < 3; d++) out[3 * (size_t)id[s] + d] = soa[(size_t)d * n + s];
+    };
+    std::vector<double> soa(3 * (size_t)n);
+    switch (field) {
+      case OMDG_X:
+        OMDG_HIP(hipMemcpy(soa.data(), c.x[c.cur], sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+        scatter3(soa);
+        break;
+      case OMDG_V:
+        OMDG_HIP(hipMemcpy(soa.data(), c.v[c.cur], sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+        scatter3(soa);
+        break;
+      case OMDG_F:
+        if (c.fp64) {
+          OMDG_HIP(hipMemcpy(soa.data(), c.f64, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+        } else {
+          std::vector<float> f(3 * (size_t)n);
+          OMDG_HIP(hipMemcpy(f.data(), c.f32, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
+          for (size_t i = 0; i < f.size(); i++) soa[i] = f[i];
+        }
+        scatter3(soa);
+        break;
+      case OMDG_POT:
+      case OMDG_VIR:
+      case OMDG_RHO: {
+        std::vector<double> a(n);
+        if (field == OMDG_RHO) {
+          std::vector<float> f(n);
+          OMDG_HIP(hipMemcpy(f.data(), c.rho, sizeof(float) * n, hipMemcpyDeviceToHost));
+          for (int i = 0; i < n; i++) a[i] = f[i];
+        } else if (c.fp64) {
+          OMDG_HIP(hipMemcpy(a.data(), field == OMDG_POT ? c.pot64 : c.vir64, sizeof(double) * n,
+                             hipMemcpyDeviceToHost));
+        } else {
+          std::vector<float> f(n);
+          OMDG_HIP(hipMemcpy(f.data(), field == OMDG_POT ? c.pot32 : c.vir32, sizeof(float) * n,
+                             hipMemcpyDeviceToHost));
+          for (int i = 0; i < n; i++) a[i] = f[i];
+        }
+        for (int s = 0; s < n; s++) out[id[s]] = a[s];
+        break;
+      }
+      default:
+        throw ApiFail(OMDG_EINVAL, "unknown field");
+    }
+  });
+}
+
+int omdg_add_table(omdg_t h, int kind, int ta, int tb, int n, double dr, double r0, int rmult,
+                   const double* y) {
+  if (!h || !y || n < 3 || !(dr > 0) || kind < 0 || kind > 2 || ta < 0 || tb < 0 ||
+      ta >= kMaxTypes || tb >= kMaxTypes) {
+    set_error("bad argument to omdg_add_table");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    std::vector<double> abcd;
+    spline_coefficients(n, dr, y, abcd);
+    DevTable t;
+    t.n = n;
+    t.rmult = rmult ? 1 : 0;
+    t.dr = dr;
+    t.r0 = r0;
+    t.dr_f = (float)dr;
+    t.inv_dr_f = (float)(1.0 / dr);
+    t.r0_f = (float)r0;
+    t.off = (int)(c.host_c64.size() / 4);
+    c.host_c64.insert(c.host_c64.end(), abcd.begin(), abcd.end());
+    int id = (int)c.tables.size();
+    c.tables.push_back(t);
+    if (kind == OMDG_TAB_EMBED) {
+      c.tab_index[kind][ta][ta] = id;
+    } else {
+      c.tab_index[kind][ta][tb] = id;
+      c.tab_index[kind][tb][ta] = id;
+    }
+    c.tables_dirty = true;
+    if (c.potential != OMDG_POT_NONE) update_cut(c);
+  });
+}
+
+int omdg_load_table(omdg_t h, int kind, int ta, int tb, const char* path, const char* name) {
+  if (!h || !path || !name) {
+    set_error("null argument");
+    return OMDG_EINVAL;
+  }
+  std::vector<double> y;
+  double dr = 0, r0 = 0;
+  int rmult = 0;
+  int rc = guarded([&] { read_table_file(path, name, y, dr, r0, rmult); });
+  if (rc) return rc;
+  return omdg_add_table(h, kind, ta, tb, (int)y.size(), dr, r0, rmult, y.data());
+}
+
+int omdg_set_potential(omdg_t h, int potential, int fp64) {
+  if (!h || potential < OMDG_POT_PAIR || potential > OMDG_POT_TERSOFF) {
+    set_error("bad potential");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    if (potential == OMDG_POT_EAM) {
+      if (c.tab_index[OMDG_TAB_PAIR][0][0] < 0 || c.tab_index[OMDG_TAB_EDENS][0][0] < 0 ||
+          c.tab_index[OMDG_TAB_EMBED][0][0] < 0)
+        throw ApiFail(OMDG_EINVAL, "EAM needs PAIR, EDENS and EMBED tables for type 0");
+      if (fp64) throw ApiFail(OMDG_EINVAL, "EAM runs with FP32 forces and FP64 integration");
+    }
+    if (potential == OMDG_POT_PAIR && c.tab_index[OMDG_TAB_PAIR][0][0] < 0) {
+      bool any = false;
+      for (auto& a : c.tab_index[OMDG_TAB_PAIR])
+        for (int b : a) any |= b >= 0;
+      if (!any) throw ApiFail(OMDG_EINVAL, "pair potential needs a PAIR table");
+    }
+    if ((potential == OMDG_POT_SW || potential == OMDG_POT_TERSOFF) && fp64)
+      throw ApiFail(OMDG_EINVAL, "three-body kernels compute FP32 forces");
+    c.potential = potential;
+    c.fp64 = fp64 ? 1 : 0;
+    update_cut(c);
+    c.list_valid = false;
+    c.forces_valid = false;
+  });
+}
+
+int omdg_set_sw(omdg_t h, const double* p) {
+  if (!h || !p) {
+    set_error("null argument");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    SwParams& s = c.sw;
+    s.eps = (float)p[0]; s.A = (float)p[1]; s.B = (float)p[2]; s.p = (float)p[3];
+    s.q = (float)p[4]; s.sigma = (float)p[5]; s.a = (float)p[6]; s.lambda = (float)p[7];
+    s.gamma = (float)p[8]; s.icos0 = (float)p[9]; s.usezbl = p[10] != 0.0;
+    s.sx0 = (float)p[11]; s.sx1 = (float)p[12]; s.sa0 = (float)p[13]; s.sa1 = (float)p[14];
+    s.sa2 = (float)p[15]; s.sa3 = (float)p[16]; s.z1 = (float)p[17]; s.z2 = (float)p[18];
+    if (c.potential == OMDG_POT_SW) update_cut(c);
+  });
+}
+
+int omdg_set_tersoff(omdg_t h, const double* p) {
+  if (!h || !p) {
+    set_error("null argument");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    TersoffParams& t = c.ters;
+    t.A = (float)p[0]; t.B = (float)p[1]; t.lambda1 = (float)p[2]; t.lambda2 = (float)p[3];
+    t.lambda3 = (float)p[4]; t.alpha = (float)p[5]; t.beta = (float)p[6]; t.n = (float)p[7];
+    t.c = (float)p[8]; t.d = (float)p[9]; t.h = (float)p[10]; t.R = (float)p[11];
+    t.D = (float)p[12]; t.m = 3;
+    if (c.potential == OMDG_POT_TERSOFF) update_cut(c);
+  });
+}
+
+int omdg_set_neighbor(omdg_t h, double skin, int check_every, int maxnbr) {
+  if (!h || skin < 0 || check_every < 1 || maxnbr < 0) {
+    set_error("bad neighbour-list parameters");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    c.skin = skin;
+    c.check_every = check_every;
+    c.maxnbr_req = maxnbr;
+    if (maxnbr > 0) c.maxnbr = maxnbr;
+    c.list_valid = false;
+  });
+}
+
+int omdg_neighbor_build(omdg_t h) {
+  if (!h) {
+    set_error("null context");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    OMDG_HIP(hipSetDevice(c.device));
+    check_ready(c);
+    rebuild(c);
+  });
+}
+
+int omdg_neighbor_export(omdg_t h, int maxn, int* cnt, int* code) {
+  if (!h || !cnt || !code || maxn <= 0) {
+    set_error("bad argument");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    if (!c.list_valid) throw ApiFail(OMDG_EINVAL, "no neighbour list built");
+    OMDG_HIP(hipStreamSynchronize(c.stream));
+    const int n = c.n;
+    std::vector<int> id(n), nn(n), cell(n);
+    std::vector<uint32_t> lst((size_t)c.maxnbr * n);
+    OMDG_HIP(hipMemcpy(id.data(), c.id[c.cur], sizeof(int) * n, hipMemcpyDeviceToHost));
+    OMDG_HIP(hipMemcpy(nn.data(), c.nnbr, sizeof(int) * n, hipMemcpyDeviceToHost));
+    OMDG_HIP(hipMemcpy(cell.data(), c.cellid, sizeof(int) * n, hipMemcpyDeviceToHost));
+    OMDG_HIP(hipMemcpy(lst.data(), c.nbr, sizeof(uint32_t) * lst.size(), hipMemcpyDeviceToHost));
+    const Grid& g = c.grid;
+    bool ovf = false;
+    for (int s = 0; s < n; s++) {
+      int i = id[s];
+      int k[3] = {cell[s] % g.M[0], (cell[s] / g.M[0]) % g.M[1], cell[s] / (g.M[0] * g.M[1])};
+      std::vector<int> row;
+      row.reserve(nn[s]);
+      for (int e = 0; e < nn[s]; e++) {
+        uint32_t w = lst[(size_t)e * n + s];
+        int t = (int)(w & kSlotMask), img = (int)(w >> kImgShift);
+        int o[3] = {img % 3 - 1, (img / 3) % 3 - 1, img / 9 - 1};
+        int sh[3];
+        for (int d = 0; d < 3; d++) {
+          int kk = k[d] + o[d];
+          sh[d] = ((g.pbc >> d) & 1) ? (kk < 0 ? -1 : (kk >= g.M[d] ? 1 : 0)) : 0;
+        }
+        row.push_back(id[t] * 27 + (sh[0] + 1) + 3 * (sh[1] + 1) + 9 * (sh[2] + 1));
+      }
+      std::sort(row.begin(), row.end());
+      cnt[i] = (int)row.size();
+      if ((int)row.size() > maxn) ovf = true;
+      for (int e = 0; e < (int)row.size() && e < maxn; e++) code[(size_t)i * maxn + e] = row[e];
+    }
+    if (ovf) throw ApiFail(OMDG_EINVAL, "export buffer too small");
+  });
+}
+
+int omdg_compute(omdg_t h, double* epot, double* virial) {
+  if (!h) {
+    set_error("null context");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    OMDG_HIP(hipSetDevice(c.device));
+    check_ready(c);
+    ensure_list(c);
+    compute_forces(c, true);
+    reduce_energy(c, false, true);
+    read_status(c);
+    raise_device_errors(c);
+    if (epot) *epot = c.epot;
+    if (virial) *virial = c.virial;
+  });
+}
+
+int omdg_zero_forces(omdg_t h) {
+  if (!h) {
+    set_error("null context");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    if (c.n <= 0) throw ApiFail(OMDG_EINVAL, "no atoms uploaded");
+    zero_forces(c);
+    c.forces_valid = true;
+  });
+}
+
+int omdg_step(omdg_t h, int nsteps, double dt, double* ekin, double* epot) {
+  if (!h || nsteps < 0 || !(dt > 0)) {
+    set_error("bad argument to omdg_step");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    OMDG_HIP(hipSetDevice(c.device));
+    check_ready(c);
+    ensure_list(c);
+    if (!c.forces_valid) {
+      // the first half kick needs forces of the current positions
+      compute_forces(c, false);
+    }
+    for (int s = 0; s < nsteps; s++) {
+      {
+        TimerScope ts(c, TG_DRIFT);
+        integrate_kick_drift(c, dt);
+      }
+      if ((s + 1) % c.check_every == 0) {
+        read_status(c);
+        raise_device_errors(c);
+        const double disp = std::sqrt((double)bits_to_float(c.hstat[ST_DISP2]));
+        const double vmax = std::sqrt((double)bits_to_float(c.hstat[ST_VMAX2]));
+        // rebuild before any atom can cover skin/2 before the next check
+        if (disp + 1.5 * c.check_every * vmax * dt >= 0.5 * c.skin) rebuild(c);
+      }
+      const bool last = (s == nsteps - 1);
+      compute_forces(c, last);
+      TimerScope ts(c, TG_KICK);
+      integrate_kick(c, dt);
+    }
+    {
+      TimerScope ts(c, TG_REDUCE);
+      reduce_energy(c, true, nsteps > 0);
+    }
+    read_status(c);
+    raise_device_errors(c);
+    if (ekin) *ekin = c.ekin;
+    if (epot) *epot = c.epot;
+  });
+}
+
+int omdg_timing(omdg_t h, int enable, int max, double* ms, int* count) {
+  if (!h) {
+    set_error("null context");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    if (enable) {
+      c.ev_used = 0;
+      c.ev_group.clear();
+      c.timing = true;
+      return;
+    }
+    OMDG_HIP(hipStreamSynchronize(c.stream));
+    std::vector<double> acc(TG_COUNT, 0.0);
+    std::vector<int> cnt(TG_COUNT, 0);
+    for (size_t k = 0; k < c.ev_group.size(); k++) {
+      float t = 0.f;
+      OMDG_HIP(hipEventElapsedTime(&t, c.ev_pool[2 * k], c.ev_pool[2 * k + 1]));
+      acc[c.ev_group[k]] += t;
+      cnt[c.ev_group[k]]++;
+    }
+    c.timing = false;
+    for (int g = 0; g < max && g < TG_COUNT; g++) {
+      if (ms) ms[g] = acc[g];
+      if (count) count[g] = cnt[g];
+    }
+  });
+}
+
+}  // extern "C"

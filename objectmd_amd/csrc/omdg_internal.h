@@ -1,0 +1,203 @@
+// Internal declarations of the MI355X force engine (device context, kernel
+// parameter blocks, launchers).  Not part of the public C ABI (include/omdg.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/omdg.h"
+
+namespace omdg {
+
+constexpr int kMaxTypes = 4;
+constexpr int kMaxTables = 3 * kMaxTypes * kMaxTypes;
+constexpr int kImgShift = 27;                   // list entry = slot | img << 27
+constexpr uint32_t kSlotMask = (1u << kImgShift) - 1u;
+
+// device status words (ctx->dstat)
+enum : int {
+  ST_ERR = 0,        // error bits, see ERR_*
+  ST_DISP2 = 1,      // max |x - x_build|^2 since the last build (float bits)
+  ST_VMAX2 = 2,      // max |v|^2 seen by the drift (float bits)
+  ST_MAXNBR = 3,     // largest neighbour count of the last build
+  ST_COUNT = 8
+};
+enum : int {
+  ERR_RANGE_HI = 1,
+  ERR_RANGE_LO = 2,
+  ERR_OVERLAP = 4,
+  ERR_NBR_OVERFLOW = 8,
+  ERR_SKIN = 16,
+  ERR_WRAP = 32,
+};
+
+// One spline table on the device.  Coefficients {a,b,c,d} per knot.
+struct DevTable {
+  int n = 0;
+  int rmult = 0;
+  double dr = 0, r0 = 0;
+  float inv_dr_f = 0, dr_f = 0, r0_f = 0;
+  int off = -1;  // offset (in knots) into the packed coefficient arrays
+};
+
+// Kernel view of a table (passed by value).
+struct TabView {
+  int n;
+  int rmult;
+  float r0, dr, inv_dr;
+  double r0d, drd;
+  const float4* c32;   // {a,b,c,d}
+  const double4* c64;  // {a,b,c,d}
+};
+
+// Geometry of the cell grid of the last build.
+struct Grid {
+  int M[3] = {1, 1, 1};
+  double origin[3] = {0, 0, 0};
+  double h[3] = {1, 1, 1};
+  double L[3] = {0, 0, 0};
+  int pbc = 0;
+  long ncell() const { return (long)M[0] * M[1] * M[2]; }
+};
+
+struct SwParams {
+  float eps, A, B, p, q, sigma, a, lambda, gamma, icos0;
+  int usezbl;
+  float sx0, sx1, sa0, sa1, sa2, sa3;
+  float z1, z2;
+};
+
+struct TersoffParams {
+  float A, B, lambda1, lambda2, lambda3, alpha, beta, n, c, d, h, R, D;
+  int m;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // box
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, L[3] = {0, 0, 0};
+  int pbc = 0;
+  bool box_set = false;
+  // atoms
+  int n = 0, ntypes = 1;
+  double mass[kMaxTypes] = {1, 1, 1, 1};
+  bool has_flags = false;
+  // device per-atom arrays (slot order), double-buffered where permuted
+  double* x[2] = {nullptr, nullptr};  // 3 SoA planes of n doubles each
+  double* v[2] = {nullptr, nullptr};
+  int* type[2] = {nullptr, nullptr};
+  int* flag[2] = {nullptr, nullptr};
+  int* id[2] = {nullptr, nullptr};  // slot -> original index
+  int cur = 0;
+  double* f64 = nullptr;  // 3 planes (FP64 force mode)
+  float* f32 = nullptr;   // 3 planes (FP32 force mode)
+  float4* q = nullptr;    // cell-relative position, w = type bits
+  float4* qb = nullptr;   // q at the last build
+  double* xb = nullptr;   // FP64 positions at the last build (FP64 mode)
+  int* cellid = nullptr;
+  int* cellid2 = nullptr;
+  float* pot32 = nullptr;
+  float* vir32 = nullptr;
+  double* pot64 = nullptr;
+  double* vir64 = nullptr;
+  float* rho = nullptr;
+  float* fp = nullptr;  // F'(rho)
+  float* emb = nullptr; // F(rho)
+  int cap_atoms = 0;
+  // neighbour list (ELL, column-major: nbr[k * n + i])
+  uint32_t* nbr = nullptr;
+  int* nnbr = nullptr;
+  int maxnbr = 0;
+  int maxnbr_req = 0;
+  long cap_nbr = 0;
+  double skin = 0.3;
+  int check_every = 10;
+  double rlist = 0;
+  bool list_valid = false;
+  Grid grid;
+  int* cell_start = nullptr;
+  int* cell_fill = nullptr;
+  int* perm = nullptr;
+  long cap_cells = 0;
+  double* aabb = nullptr;  // device scratch for min/max (6 doubles)
+  // status
+  int* dstat = nullptr;
+  int* hstat = nullptr;  // pinned
+  double* dred = nullptr;  // reduction partials
+  double* dres = nullptr;  // reduction results (8 doubles)
+  double* hres = nullptr;  // pinned mirror
+  int red_blocks = 0;
+  // potential
+  int potential = OMDG_POT_NONE;
+  int fp64 = 0;
+  std::vector<DevTable> tables;
+  std::vector<double> host_c64;  // packed coefficients (host mirror)
+  float* tab32 = nullptr;
+  double* tab64 = nullptr;
+  bool tables_dirty = true;
+  int tab_index[3][kMaxTypes][kMaxTypes];  // kind -> (ta,tb) -> table id
+  SwParams sw{};
+  TersoffParams ters{};
+  double cut = 0;  // largest interaction cutoff
+  // results of the last evaluation
+  double epot = 0, virial = 0, ekin = 0;
+  bool forces_valid = false;
+  // kernel timing (omdg_timing)
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<int> ev_group;  // group of each recorded (start, end) pair
+};
+
+enum : int { TG_BUILD = 0, TG_PASS1 = 1, TG_FORCE = 2, TG_DRIFT = 3, TG_KICK = 4,
+             TG_REDUCE = 5, TG_COUNT = 6 };
+
+// Brackets the launches of one group with HIP events when timing is on.
+struct TimerScope {
+  Ctx& c;
+  int group;
+  bool on;
+  TimerScope(Ctx& ctx, int g);
+  ~TimerScope();
+};
+
+// ---- launchers (defined in the .hip files) --------------------------------
+void nbr_build(Ctx& c);
+void eam_forces(Ctx& c, bool energy);
+void pair_forces(Ctx& c, bool energy);
+void sw_forces(Ctx& c, bool energy);
+void tersoff_forces(Ctx& c, bool energy);
+void integrate_kick_drift(Ctx& c, double dt);
+void integrate_kick(Ctx& c, double dt);
+void reduce_energy(Ctx& c, bool kinetic, bool potential);
+void zero_forces(Ctx& c);
+
+// ---- helpers ---------------------------------------------------------------
+void set_error(const std::string& msg);
+TabView tab_view(const Ctx& c, int tid);
+int grid_blocks(long n, int block);
+
+}  // namespace omdg
+
+#define OMDG_HIP(call)                                                  \
+  do {                                                                  \
+    hipError_t _e = (call);                                             \
+    if (_e != hipSuccess)                                               \
+      throw omdg::HipFail(std::string(#call) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+namespace omdg {
+struct HipFail {
+  std::string msg;
+  explicit HipFail(std::string m) : msg(std::move(m)) {}
+};
+struct ApiFail {
+  int code;
+  std::string msg;
+  ApiFail(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+}  // namespace omdg

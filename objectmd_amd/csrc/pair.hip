@@ -1,0 +1,219 @@
+// Tabulated pair potential on the full neighbour list (gather form).
+// Replaces TForcePair::ComputeHalf (include/potential/tpair.h:62-71):
+// r < CutRadius = table max range, fr = -V'(r)/r, half the pair energy and
+// the full pair virial to the atom (ForceKernel::ReturnForce,
+// include/omd/forcekernel.h:92-110).
+//
+// FP64 mode ("plain method"): the displacement is formed exactly as the
+// reference forms it, x_i - (x_j + image shift), from FP64 positions, and the
+// spline is read in FP64.  FP32 mode uses the cell-relative snapshots.
+#include <hip/hip_runtime.h>
+
+#include "omdg_internal.h"
+#include "spline_dev.h"
+
+namespace omdg {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct PairArgs {
+  const double* x;  // 3 planes (FP64 mode)
+  const float4* q;  // FP32 mode
+  const int* type;
+  const int* cellid;
+  const uint32_t* nbr;
+  const int* nnbr;
+  double* f64;
+  double* pot64;
+  double* vir64;
+  float* f32;
+  float* pot32;
+  float* vir32;
+  int* err;
+  int n;
+  int M[3];
+  int pbc;
+  double L[3];
+  float hx, hy, hz;
+  int ntypes;
+  signed char tab[kMaxTypes * kMaxTypes];  // table slot per type pair, -1 none
+  double cut[kMaxTypes * kMaxTypes];       // table max range
+  TabView tabs[kMaxTypes * kMaxTypes];
+};
+
+__device__ inline double shifted(double x, int s, double L) {
+  return s == 0 ? x : (s < 0 ? x + (-L) : x + L);
+}
+
+template <bool ENERGY>
+__global__ void __launch_bounds__(kBlock) k_pair64(PairArgs a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  const int n = a.n;
+  const double xi = a.x[s], yi = a.x[(size_t)n + s], zi = a.x[2 * (size_t)n + s];
+  const int c = a.cellid[s];
+  const int k[3] = {c % a.M[0], (c / a.M[0]) % a.M[1], c / (a.M[0] * a.M[1])};
+  const int ti = a.type[s];
+  const int nn = a.nnbr[s];
+  double fx = 0, fy = 0, fz = 0, pot = 0, vir = 0;
+  for (int e = 0; e < nn; e++) {
+    const uint32_t w = a.nbr[(size_t)e * n + s];
+    const int t = (int)(w & kSlotMask);
+    const int img = (int)(w >> kImgShift);
+    const int o[3] = {img % 3 - 1, (img / 3) % 3 - 1, img / 9 - 1};
+    int sh[3];
+    for (int d = 0; d < 3; d++) {
+      const int kk = k[d] + o[d];
+      sh[d] = ((a.pbc >> d) & 1) ? (kk < 0 ? -1 : (kk >= a.M[d] ? 1 : 0)) : 0;
+    }
+    const int slot = ti * kMaxTypes + a.type[t];
+    const int tb = a.tab[slot];
+    if (tb < 0) continue;
+    double dx, dy, dz, r2;
+    {
+#pragma clang fp contract(off)
+      dx = xi - shifted(a.x[t], sh[0], a.L[0]);
+      dy = yi - shifted(a.x[(size_t)n + t], sh[1], a.L[1]);
+      dz = zi - shifted(a.x[2 * (size_t)n + t], sh[2], a.L[2]);
+      r2 = dx * dx + dy * dy + dz * dz;
+    }
+    if (r2 < 1e-6) {
+      atomicOr(a.err, ERR_OVERLAP);
+      continue;
+    }
+    const double r = sqrt(r2);
+    if (r < a.cut[slot]) {
+      double p, dp;
+      spline_read64(a.tabs[tb], r, p, dp, a.err);
+      const double ff = -dp / r;
+      const double gx = dx * ff, gy = dy * ff, gz = dz * ff;
+      fx += gx;
+      fy += gy;
+      fz += gz;
+      if (ENERGY) {
+        pot += 0.5 * p;
+        vir += gx * dx + gy * dy + gz * dz;
+      }
+    }
+  }
+  a.f64[s] = fx;
+  a.f64[(size_t)n + s] = fy;
+  a.f64[2 * (size_t)n + s] = fz;
+  if (ENERGY) {
+    a.pot64[s] = pot;
+    a.vir64[s] = vir;
+  }
+}
+
+template <bool ENERGY>
+__global__ void __launch_bounds__(kBlock) k_pair32(PairArgs a) {
+  __shared__ float4 offs[27];
+  load_offsets(offs, a.hx, a.hy, a.hz);
+  __syncthreads();
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  const int n = a.n;
+  const float4 qi = a.q[s];
+  const int ti = __float_as_int(qi.w);
+  const int nn = a.nnbr[s];
+  Kahan fx, fy, fz;
+  float pot = 0.f, vir = 0.f;
+  for (int e = 0; e < nn; e++) {
+    const uint32_t w = a.nbr[(size_t)e * n + s];
+    const float4 qj = a.q[w & kSlotMask];
+    const int slot = ti * kMaxTypes + __float_as_int(qj.w);
+    const int tb = a.tab[slot];
+    if (tb < 0) continue;
+    const float4 o = offs[w >> kImgShift];
+    const float dx = qi.x - qj.x - o.x;
+    const float dy = qi.y - qj.y - o.y;
+    const float dz = qi.z - qj.z - o.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 < 1e-6f) {
+      atomicOr(a.err, ERR_OVERLAP);
+      continue;
+    }
+    const float r = sqrtf(r2);
+    if (r < (float)a.cut[slot]) {
+      float p, dp;
+      spline_read32(a.tabs[tb], r, p, dp, a.err);
+      const float ff = -dp / r;
+      fx.add(ff * dx);
+      fy.add(ff * dy);
+      fz.add(ff * dz);
+      if (ENERGY) {
+        pot += 0.5f * p;
+        vir += ff * r2;
+      }
+    }
+  }
+  a.f32[s] = fx.s;
+  a.f32[(size_t)n + s] = fy.s;
+  a.f32[2 * (size_t)n + s] = fz.s;
+  if (ENERGY) {
+    a.pot32[s] = pot;
+    a.vir32[s] = vir;
+  }
+}
+
+}  // namespace
+
+void pair_forces(Ctx& c, bool energy) {
+  PairArgs a{};
+  a.x = c.x[c.cur];
+  a.q = c.q;
+  a.type = c.type[c.cur];
+  a.cellid = c.cellid;
+  a.nbr = c.nbr;
+  a.nnbr = c.nnbr;
+  a.f64 = c.f64;
+  a.pot64 = c.pot64;
+  a.vir64 = c.vir64;
+  a.f32 = c.f32;
+  a.pot32 = c.pot32;
+  a.vir32 = c.vir32;
+  a.err = c.dstat + ST_ERR;
+  a.n = c.n;
+  for (int d = 0; d < 3; d++) {
+    a.M[d] = c.grid.M[d];
+    a.L[d] = c.grid.L[d];
+  }
+  a.pbc = c.grid.pbc;
+  a.hx = (float)c.grid.h[0];
+  a.hy = (float)c.grid.h[1];
+  a.hz = (float)c.grid.h[2];
+  a.ntypes = c.ntypes;
+  int used = 0;
+  for (int i = 0; i < kMaxTypes; i++)
+    for (int j = 0; j < kMaxTypes; j++) {
+      const int slot = i * kMaxTypes + j;
+      const int tid = c.tab_index[OMDG_TAB_PAIR][i][j];
+      a.tab[slot] = -1;
+      a.cut[slot] = 0;
+      if (tid < 0) continue;
+      // one TabView per table; type pairs sharing a table share the view
+      int found = -1;
+      for (int u = 0; u < used; u++)
+        if (a.tabs[u].c64 == tab_view(c, tid).c64) found = u;
+      if (found < 0) {
+        a.tabs[used] = tab_view(c, tid);
+        found = used++;
+      }
+      a.tab[slot] = (signed char)found;
+      const TabView& t = a.tabs[found];
+      a.cut[slot] = t.r0d + (double)(t.n - 1) * t.drd;
+    }
+  const int nb = grid_blocks(c.n, kBlock);
+  if (c.fp64) {
+    if (energy) k_pair64<true><<<nb, kBlock, 0, c.stream>>>(a);
+    else k_pair64<false><<<nb, kBlock, 0, c.stream>>>(a);
+  } else {
+    if (energy) k_pair32<true><<<nb, kBlock, 0, c.stream>>>(a);
+    else k_pair32<false><<<nb, kBlock, 0, c.stream>>>(a);
+  }
+  OMDG_HIP(hipGetLastError());
+}
+
+}  // namespace omdg

@@ -1,0 +1,140 @@
+// Device-side evaluation of the reference's cubic-spline tables
+// (TableReader::read, src/omd/treader.cpp:193-264), FP32 and FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "omdg_internal.h"
+
+namespace omdg {
+
+// Segment location of treader.cpp:195-221: truncating conversion, an index
+// of n-1 is folded onto the last segment with dx += dr, an index of -1
+// evaluates segment 0 at dx = rval + dr; beyond either end sets an error bit
+// and returns false.
+__device__ inline bool spline_locate32(const TabView& t, float r, int& x, float& dx,
+                                       int* err) {
+  const float rval = r - t.r0;
+  const float qf = rval * t.inv_dr;
+  if (!(qf < (float)t.n)) {  // also catches NaN
+    atomicOr(err, ERR_RANGE_HI);
+    return false;
+  }
+  if (qf <= -2.0f) {
+    atomicOr(err, ERR_RANGE_LO);
+    return false;
+  }
+  x = (int)qf;
+  dx = rval - (float)x * t.dr;
+  if (x == t.n - 1) {
+    dx += t.dr;
+    x--;
+  }
+  if (x == -1) x = 0;
+  return true;
+}
+
+__device__ inline bool spline_read32(const TabView& t, float r, float& val, float& dval,
+                                     int* err) {
+  int x;
+  float dx;
+  if (!spline_locate32(t, r, x, dx, err)) {
+    val = dval = 0.f;
+    return false;
+  }
+  const float4 c = t.c32[x];  // {a,b,c,d}
+  val = c.w + dx * (c.z + dx * (c.y + dx * c.x));
+  dval = c.z + dx * (2.0f * c.y + 3.0f * dx * c.x);
+  if (t.rmult) {
+    val /= r;
+    dval = (dval - val) / r;
+  }
+  return true;
+}
+
+__device__ inline bool spline_value32(const TabView& t, float r, float& val, int* err) {
+  int x;
+  float dx;
+  if (!spline_locate32(t, r, x, dx, err)) {
+    val = 0.f;
+    return false;
+  }
+  const float4 c = t.c32[x];
+  val = c.w + dx * (c.z + dx * (c.y + dx * c.x));
+  if (t.rmult) val /= r;
+  return true;
+}
+
+__device__ inline bool spline_locate64(const TabView& t, double r, int& x, double& dx,
+                                       int* err) {
+  const double rval = r - t.r0d;
+  const double q = rval / t.drd;
+  if (!(q < (double)t.n)) {
+    atomicOr(err, ERR_RANGE_HI);
+    return false;
+  }
+  if (q <= -2.0) {
+    atomicOr(err, ERR_RANGE_LO);
+    return false;
+  }
+  x = (int)q;
+  dx = rval - (double)x * t.drd;
+  if (x == t.n - 1) {
+    dx += t.drd;
+    x--;
+  }
+  if (x == -1) x = 0;
+  return true;
+}
+
+__device__ inline bool spline_read64(const TabView& t, double r, double& val, double& dval,
+                                     int* err) {
+  int x;
+  double dx;
+  if (!spline_locate64(t, r, x, dx, err)) {
+    val = dval = 0.0;
+    return false;
+  }
+  const double4 c = t.c64[x];
+  val = c.w + dx * (c.z + dx * (c.y + dx * c.x));
+  dval = c.z + dx * (2.0 * c.y + 3.0 * dx * c.x);
+  if (t.rmult) {
+    val /= r;
+    dval = (dval - val) / r;
+  }
+  return true;
+}
+
+__device__ inline bool spline_value64(const TabView& t, double r, double& val, int* err) {
+  int x;
+  double dx;
+  if (!spline_locate64(t, r, x, dx, err)) {
+    val = 0.0;
+    return false;
+  }
+  const double4 c = t.c64[x];
+  val = c.w + dx * (c.z + dx * (c.y + dx * c.x));
+  if (t.rmult) val /= r;
+  return true;
+}
+
+// Compensated (Kahan) accumulation of FP32 terms.
+struct Kahan {
+  float s = 0.f, c = 0.f;
+  __device__ inline void add(float v) {
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
+// Per-block table of stencil offsets o * h for the 27 image codes.
+__device__ inline void load_offsets(float4* offs, float hx, float hy, float hz) {
+  for (int i = threadIdx.x; i < 27; i += blockDim.x) {
+    const int ox = i % 3 - 1, oy = (i / 3) % 3 - 1, oz = i / 9 - 1;
+    offs[i] = make_float4((float)ox * hx, (float)oy * hy, (float)oz * hz, 0.f);
+  }
+}
+
+}  // namespace omdg

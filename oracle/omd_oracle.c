@@ -1,0 +1,1055 @@
+/*
+ * FP64 CPU oracle for the Object-MD force/integration path.
+ *
+ * TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the CPU
+ * baseline leg of bench.py as the checker; never linked into or called by
+ * the product library.  Compiled with -ffp-contract=off so every product and
+ * sum is rounded exactly as written, like the reference's x86-64 build.
+ *
+ * What pins each function (tests/test_oracle_*.py):
+ *   orc_spline_coef/read   quadratic data reproduced exactly (parabolic
+ *                          run-out), interpolation at the knots, C1/C2
+ *                          continuity, cubic-segment identity, RMULT
+ *                          derivative identity, edge branches x==-1/x==n-1
+ *                          and the range errors of treader.cpp:193-229.
+ *   orc_table_load         the reference's own tables/ files (headers and
+ *                          first/last samples, parsed from the text).
+ *   orc_fcc100/diamond100  atom counts, nearest-neighbour distances and the
+ *                          box of Crystalline::CalcBox for known sizes.
+ *   orc_nbr_full           brute-force O(N^2 x 27 images) enumeration; pair
+ *                          symmetry; fcc/diamond shell counts.
+ *   orc_pair_forces        analytic Lennard-Jones, fcc lattice sums, Newton's
+ *                          third law, central finite differences of Ep.
+ *   orc_eam_forces         finite differences of Ep, sum f = 0, zero force in
+ *                          the ideal crystal, virial = -sum r.f, analytic
+ *                          Johnson functions through the spline.
+ *   orc_sw_forces          ideal diamond at a=5.431 gives -2 eps per atom,
+ *                          finite differences, sum f = 0, brute-force triplet
+ *                          counts on the reference's small clusters.
+ *   orc_tersoff_forces     Si(B) ideal diamond cohesive energy -4.63 eV/atom
+ *                          (PRB 37 6991), finite differences, sum f = 0.
+ *   orc_vv_* / orc_run     harmonic dimer closed form, time reversibility,
+ *                          energy conservation.
+ */
+#include "omd_oracle.h"
+
+#include <limits.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_PROXIMITY 1e-6 /* include/omd/config.h:31 OMD_PROXIMITY */
+
+/* ------------------------------------------------------------------------ */
+/* spline: src/omd/treader.cpp:37-64 (coefficients), :193-264 (evaluation)   */
+/* ------------------------------------------------------------------------ */
+
+/* Parabolic run-out cubic spline through equispaced samples.  With
+ * S_i = h^2 M_i / 6 the interior equations are S_{i-1} + 4 S_i + S_{i+1} =
+ * y_{i+1} - 2 y_i + y_{i-1}; the end conditions S_0 = S_1, S_{n-1} = S_{n-2}
+ * turn the first and last diagonal into 5.  Solved by the Thomas algorithm
+ * in the operation order of treader.cpp:45-58. */
+void orc_spline_coef(int n, double dr, const double *y, double *abcd) {
+    int m = n - 2;
+    for (int i = 0; i < n; i++) {
+        abcd[4 * i + 0] = abcd[4 * i + 1] = abcd[4 * i + 2] = 0.0;
+        abcd[4 * i + 3] = y[i];
+    }
+    if (m < 1) return;
+    double *dg = (double *)malloc(sizeof(double) * m);
+    double *rh = (double *)malloc(sizeof(double) * m);
+    double *S = (double *)malloc(sizeof(double) * n);
+    for (int i = 0; i < m; i++) rh[i] = y[i + 2] - 2.0 * y[i + 1] + y[i];
+    for (int i = 0; i < m; i++) dg[i] = 4.0;
+    dg[0] = 5.0;
+    dg[m - 1] = 5.0;
+    for (int k = 1; k < m; k++) {
+        double w = 1.0 / dg[k - 1];
+        dg[k] -= w;
+        rh[k] -= w * rh[k - 1];
+    }
+    S[m] = rh[m - 1] / dg[m - 1];
+    S[m + 1] = S[m];
+    for (int k = m - 2; k >= 0; k--) S[k + 1] = (rh[k] - S[k + 2]) / dg[k];
+    S[0] = S[1];
+    for (int i = 0; i < n - 1; i++) {
+        abcd[4 * i + 0] = (S[i + 1] - S[i]) / dr / dr / dr;
+        abcd[4 * i + 1] = 3.0 * S[i] / dr / dr;
+        abcd[4 * i + 2] = (y[i + 1] - y[i] - 2.0 * S[i] - S[i + 1]) / dr;
+    }
+    free(dg);
+    free(rh);
+    free(S);
+}
+
+/* Locate the segment as treader.cpp:195-221 does: truncating conversion,
+ * x == n-1 folded onto the last segment, x == -1 extrapolates segment 0. */
+static int spline_locate(const orc_table *t, double r, int *seg, double *dx) {
+    double rval = r - t->r0;
+    double q = rval / t->dr;
+    if (q >= (double)INT_MAX) return 1;
+    if (q <= (double)INT_MIN) return 2;
+    int x = (int)q;
+    double d = rval - (double)x * t->dr;
+    if (x > t->n - 1) return 1;
+    if (x < -1) return 2;
+    if (x == t->n - 1) {
+        d += t->dr;
+        x--;
+    }
+    if (x == -1) x = 0;
+    *seg = x;
+    *dx = d;
+    return 0;
+}
+
+int orc_spline_read(const orc_table *t, double r, double *val, double *dval) {
+    int x;
+    double dx;
+    int e = spline_locate(t, r, &x, &dx);
+    if (e) return e;
+    const double *c = t->abcd + 4 * x;
+    double v = c[3] + dx * (c[2] + dx * (c[1] + dx * c[0]));
+    double dv = c[2] + dx * (2.0 * c[1] + 3.0 * dx * c[0]);
+    if (t->rmult) {
+        v /= r;
+        dv = (dv - v) / r;
+    }
+    *val = v;
+    *dval = dv;
+    return 0;
+}
+
+int orc_spline_value(const orc_table *t, double r, double *val) {
+    int x;
+    double dx;
+    int e = spline_locate(t, r, &x, &dx);
+    if (e) return e;
+    const double *c = t->abcd + 4 * x;
+    double v = c[3] + dx * (c[2] + dx * (c[1] + dx * c[0]));
+    if (t->rmult) v /= r;
+    *val = v;
+    return 0;
+}
+
+/* Table file: "#$<name> key value" pseudo-comment lines, data after the
+ * "#$<name> --" line (param.cpp:106-139, treader.cpp:83-136). */
+int orc_table_load(const char *path, const char *name, int maxn, double *y,
+                   int *n, double *dr, double *r0, int *rmult) {
+    FILE *fl = fopen(path, "r");
+    if (!fl) return -1;
+    char mark[256];
+    snprintf(mark, sizeof mark, "#$%s", name);
+    char line[4096];
+    int have_n = 0, have_dr = 0, found = 0;
+    *r0 = 0.0;
+    *rmult = 0;
+    while (fgets(line, sizeof line, fl)) {
+        char *save = NULL;
+        char *tok = strtok_r(line, " \t\r\n", &save);
+        if (!tok || strcmp(tok, mark) != 0) continue;
+        char *key = strtok_r(NULL, " \t\r\n", &save);
+        if (!key) continue;
+        if (strcmp(key, "--") == 0) {
+            found = 1;
+            break;
+        }
+        char *val = strtok_r(NULL, " \t\r\n", &save);
+        if (!val) continue;
+        if (strcmp(key, "NumberOfData") == 0) {
+            *n = atoi(val);
+            have_n = 1;
+        } else if (strcmp(key, "Spacing") == 0) {
+            *dr = strtod(val, NULL);
+            have_dr = 1;
+        } else if (strcmp(key, "Offset") == 0) {
+            *r0 = strtod(val, NULL);
+        } else if (strcmp(key, "Format") == 0) {
+            if (strcmp(val, "RMULT") == 0)
+                *rmult = 1;
+            else if (strcmp(val, "PLAIN") == 0)
+                *rmult = 0;
+            else {
+                fclose(fl);
+                return -2;
+            }
+        }
+    }
+    if (!found || !have_n || !have_dr || *n > maxn || *n < 3) {
+        fclose(fl);
+        return -3;
+    }
+    for (int i = 0; i < *n; i++) {
+        if (fscanf(fl, "%lf", &y[i]) != 1) {
+            fclose(fl);
+            return -4;
+        }
+    }
+    fclose(fl);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* lattices                                                                  */
+/* ------------------------------------------------------------------------ */
+
+static void calc_box(int na, const double *xyz, double mlx, double mly,
+                     double mlz, double *box6) {
+    /* AtomContainer::CalcBox (container.cpp:174-) + Crystalline::CalcBox
+     * half-monolayer margins (Crystalline.h:53-74). */
+    double mn[3] = {1e308, 1e308, 1e308}, mx[3] = {-1e308, -1e308, -1e308};
+    for (int i = 0; i < na; i++)
+        for (int d = 0; d < 3; d++) {
+            double v = xyz[3 * i + d];
+            if (v < mn[d]) mn[d] = v;
+            if (v > mx[d]) mx[d] = v;
+        }
+    box6[0] = mn[0] - 0.5 * mlx;
+    box6[1] = mx[0] + 0.5 * mlx;
+    box6[2] = mn[1] - 0.5 * mly;
+    box6[3] = mx[1] + 0.5 * mly;
+    box6[4] = mn[2] - 0.5 * mlz;
+    box6[5] = mx[2] + 0.5 * mlz;
+}
+
+/* FCC "100": unit cells iterated z, y, x, four atoms per cell, z growing
+ * downward from 0 (src/gadget/FCC.cpp:142-242). */
+int orc_fcc100(int xml, int yml, int zml, double a, double *xyz, double *box6) {
+    double h = a / 2.;
+    double rel[4][3] = {{0, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+    double x0 = 0.0, y0 = 0.0, z1 = 0.0;
+    double x1 = x0 + h * ((double)xml - 1.);
+    double y1 = y0 + h * ((double)yml - 1.);
+    double z0 = z1 - h * ((double)zml - 1.);
+    int nux = (xml + 1) / 2, nuy = (yml + 1) / 2, nuz = (zml + 1) / 2;
+    double dev = 0.0001 * h;
+    double off = 2. * h;
+    int cnt = 0;
+    for (int i = 0; i < nuz; i++) {
+        double zo = (double)i * off + z0;
+        for (int j = 0; j < nuy; j++) {
+            double yo = (double)j * off + y0;
+            for (int k = 0; k < nux; k++) {
+                double xo = (double)k * off + x0;
+                for (int l = 0; l < 4; l++) {
+                    double px = xo + rel[l][0] * h;
+                    double py = yo + rel[l][1] * h;
+                    double pz = zo + rel[l][2] * h;
+                    if (px <= x1 + dev && py <= y1 + dev && pz <= z1 + dev) {
+                        if (xyz) {
+                            xyz[3 * cnt + 0] = px;
+                            xyz[3 * cnt + 1] = py;
+                            xyz[3 * cnt + 2] = pz;
+                        }
+                        cnt++;
+                    }
+                }
+            }
+        }
+    }
+    if (xyz && box6) calc_box(cnt, xyz, h, h, h, box6);
+    return cnt;
+}
+
+/* Diamond "100": xml/4+1 cells per axis, eight atoms per cell, cells
+ * iterated x, y, z with z going down (include/crystal/Diamond.h:17-58). */
+int orc_diamond100(int xml, int yml, int zml, double a, double *xyz,
+                   double *box6) {
+    int ncx = xml / 4 + 1, ncy = yml / 4 + 1, ncz = zml / 4 + 1;
+    int na = ncx * ncy * ncz * 8;
+    if (!xyz) return na;
+    double *p = xyz;
+    double bx = 0.0;
+    for (int i = 0; i < ncx; i++) {
+        double by = 0.0;
+        for (int j = 0; j < ncy; j++) {
+            double bz = 0.0;
+            for (int k = 0; k < ncz; k++) {
+                double s[8][3] = {
+                    {bx, by, bz},
+                    {bx + a / 2., by + a / 2., bz},
+                    {bx + a / 4., by + a / 4., bz - a / 4.},
+                    {bx + 3. * a / 4., by + 3. * a / 4., bz - a / 4.},
+                    {bx, by + a / 2., bz - a / 2.},
+                    {bx + a / 2., by, bz - a / 2.},
+                    {bx + a / 4., by + 3. * a / 4., bz - 3. * a / 4.},
+                    {bx + 3. * a / 4., by + a / 4., bz - 3. * a / 4.}};
+                memcpy(p, s, sizeof s);
+                p += 24;
+                bz -= a;
+            }
+            by += a;
+        }
+        bx += a;
+    }
+    if (box6) calc_box(na, xyz, a / 4.0, a / 4.0, a / 4.0, box6);
+    return na;
+}
+
+/* ------------------------------------------------------------------------ */
+/* neighbour lists                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* Image shift of the reference: the ghost coordinate is formed first as
+ * x + (+-L) (comhandler.cpp:288-355), then dx = x_at - x_to
+ * (gadget.cpp:138-145); r^2 <= Rsq is inclusive (VerletList.cpp:184). */
+static inline double shifted(double x, int s, double L) {
+    return s == 0 ? x : (s < 0 ? x + (-L) : x + L);
+}
+
+static inline int img_code(int sx, int sy, int sz) {
+    return (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+}
+
+static int cmp_int(const void *a, const void *b) {
+    int x = *(const int *)a, y = *(const int *)b;
+    return (x > y) - (x < y);
+}
+
+static inline int active(const int *flag, int i) {
+    return flag ? (flag[i] & 1) : 1;
+}
+
+static void push(int i, int c, int maxn, int *cnt, int *code, int *ovf) {
+    if (cnt[i] < maxn)
+        code[(long)i * maxn + cnt[i]] = c;
+    else
+        *ovf = 1;
+    cnt[i]++;
+}
+
+int orc_nbr_brute(int n, const double *xyz, const int *flag, const double *lo,
+                  const double *L, int pbc, double rlist, int maxn, int *cnt,
+                  int *code) {
+    (void)lo;
+    double Rsq = rlist * rlist;
+    int ovf = 0;
+    int smax[3];
+    for (int d = 0; d < 3; d++) smax[d] = (pbc >> d) & 1;
+    for (int i = 0; i < n; i++) cnt[i] = 0;
+    for (int i = 0; i < n; i++) {
+        if (!active(flag, i)) continue;
+        for (int j = 0; j < n; j++) {
+            if (!active(flag, j)) continue;
+            for (int sz = -smax[2]; sz <= smax[2]; sz++)
+                for (int sy = -smax[1]; sy <= smax[1]; sy++)
+                    for (int sx = -smax[0]; sx <= smax[0]; sx++) {
+                        if (i == j && sx == 0 && sy == 0 && sz == 0) continue;
+                        double dx = xyz[3 * i] - shifted(xyz[3 * j], sx, L[0]);
+                        double dy =
+                            xyz[3 * i + 1] - shifted(xyz[3 * j + 1], sy, L[1]);
+                        double dz =
+                            xyz[3 * i + 2] - shifted(xyz[3 * j + 2], sz, L[2]);
+                        double rd = dx * dx + dy * dy + dz * dz;
+                        if (rd <= Rsq)
+                            push(i, j * 27 + img_code(sx, sy, sz), maxn, cnt,
+                                 code, &ovf);
+                    }
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        int c = cnt[i] < maxn ? cnt[i] : maxn;
+        qsort(code + (long)i * maxn, c, sizeof(int), cmp_int);
+    }
+    return ovf ? -1 : 0;
+}
+
+/* Linked cells over the box (any binning gives the same membership; the
+ * membership test is the one above).  Periodic coordinates must lie in
+ * [lo, lo+L) (orc_wrap); returns -2 otherwise. */
+int orc_nbr_full(int n, const double *xyz, const int *flag, const double *lo,
+                 const double *L, int pbc, double rlist, int maxn, int *cnt,
+                 int *code) {
+    double Rsq = rlist * rlist;
+    int M[3];
+    double cmin[3], h[3];
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) {
+            double v = xyz[3 * i + d];
+            if (((pbc >> d) & 1) && (v < lo[d] || v >= lo[d] + L[d]))
+                return -2;
+        }
+    for (int d = 0; d < 3; d++) {
+        if ((pbc >> d) & 1) {
+            cmin[d] = lo[d];
+            M[d] = (int)(L[d] / rlist);
+            if (M[d] < 1) M[d] = 1;
+            h[d] = L[d] / M[d];
+        } else {
+            double mn = 1e308, mx = -1e308;
+            for (int i = 0; i < n; i++) {
+                double v = xyz[3 * i + d];
+                if (v < mn) mn = v;
+                if (v > mx) mx = v;
+            }
+            if (n == 0) mn = mx = 0.0;
+            cmin[d] = mn;
+            M[d] = (int)((mx - mn) / rlist);
+            if (M[d] < 1) M[d] = 1;
+            h[d] = (mx - mn) / M[d];
+            if (h[d] <= 0.0) h[d] = 1.0;
+        }
+    }
+    /* small periodic grids: the stencil would visit a cell twice */
+    for (int d = 0; d < 3; d++)
+        if (((pbc >> d) & 1) && M[d] < 3)
+            return orc_nbr_brute(n, xyz, flag, lo, L, pbc, rlist, maxn, cnt,
+                                 code);
+    long nc = (long)M[0] * M[1] * M[2];
+    int *head = (int *)malloc(sizeof(int) * nc);
+    int *next = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+    int *cell = (int *)malloc(sizeof(int) * 3 * (n > 0 ? n : 1));
+    for (long c = 0; c < nc; c++) head[c] = -1;
+    for (int i = n - 1; i >= 0; i--) {
+        int cc[3];
+        for (int d = 0; d < 3; d++) {
+            int k = (int)floor((xyz[3 * i + d] - cmin[d]) / h[d]);
+            if (k < 0) k = 0;
+            if (k > M[d] - 1) k = M[d] - 1;
+            cc[d] = k;
+            cell[3 * i + d] = k;
+        }
+        long c = cc[0] + (long)M[0] * (cc[1] + (long)M[1] * cc[2]);
+        next[i] = head[c];
+        head[c] = i;
+    }
+    int ovf = 0;
+    for (int i = 0; i < n; i++) cnt[i] = 0;
+    for (int i = 0; i < n; i++) {
+        if (!active(flag, i)) continue;
+        for (int oz = -1; oz <= 1; oz++)
+            for (int oy = -1; oy <= 1; oy++)
+                for (int ox = -1; ox <= 1; ox++) {
+                    int o[3] = {ox, oy, oz}, cc[3], s[3];
+                    int skip = 0;
+                    for (int d = 0; d < 3; d++) {
+                        int k = cell[3 * i + d] + o[d];
+                        s[d] = 0;
+                        if ((pbc >> d) & 1) {
+                            if (k < 0) {
+                                k += M[d];
+                                s[d] = -1;
+                            } else if (k >= M[d]) {
+                                k -= M[d];
+                                s[d] = 1;
+                            }
+                        } else if (k < 0 || k >= M[d]) {
+                            skip = 1;
+                        }
+                        cc[d] = k;
+                    }
+                    if (skip) continue;
+                    long c = cc[0] + (long)M[0] * (cc[1] + (long)M[1] * cc[2]);
+                    for (int j = head[c]; j >= 0; j = next[j]) {
+                        if (!active(flag, j)) continue;
+                        if (i == j && !s[0] && !s[1] && !s[2]) continue;
+                        double dx = xyz[3 * i] - shifted(xyz[3 * j], s[0], L[0]);
+                        double dy =
+                            xyz[3 * i + 1] - shifted(xyz[3 * j + 1], s[1], L[1]);
+                        double dz =
+                            xyz[3 * i + 2] - shifted(xyz[3 * j + 2], s[2], L[2]);
+                        double rd = dx * dx + dy * dy + dz * dz;
+                        if (rd <= Rsq)
+                            push(i, j * 27 + img_code(s[0], s[1], s[2]), maxn,
+                                 cnt, code, &ovf);
+                    }
+                }
+    }
+    for (int i = 0; i < n; i++) {
+        int c = cnt[i] < maxn ? cnt[i] : maxn;
+        qsort(code + (long)i * maxn, c, sizeof(int), cmp_int);
+    }
+    free(head);
+    free(next);
+    free(cell);
+    return ovf ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* pair geometry helpers                                                     */
+/* ------------------------------------------------------------------------ */
+
+static inline void decode(int c, int *j, int *s) {
+    *j = c / 27;
+    int im = c % 27;
+    s[0] = im % 3 - 1;
+    s[1] = (im / 3) % 3 - 1;
+    s[2] = im / 9 - 1;
+}
+
+/* dx = x_at - (x_to + shift)  (gadget.cpp:140-142 with the ghost image) */
+static inline double pair_d(const double *xyz, const double *L, int i, int j,
+                            const int *s, double *d) {
+    for (int k = 0; k < 3; k++)
+        d[k] = xyz[3 * i + k] - shifted(xyz[3 * j + k], s[k], L[k]);
+    return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+}
+
+/* A pair (i, j, s) of the full list is visited once in half iteration:
+ * j > i, or j == i with the image in the upper half. */
+static inline int half_owner(int i, int j, int im) {
+    return j > i || (j == i && im > 13);
+}
+
+/* ForceKernel::ReturnForce (forcekernel.h:92-110): fr = f/r; half of the
+ * pair energy and the full pair virial go to each atom. */
+static inline void return_force(int i, int j, const double *d, double fr,
+                                double p, double *f, double *pot,
+                                double *vir) {
+    double fx = d[0] * fr, fy = d[1] * fr, fz = d[2] * fr;
+    double v = (fx * d[0] + fy * d[1] + fz * d[2]);
+    p *= 0.5;
+    if (vir) {
+        vir[i] += v;
+        vir[j] += v;
+    }
+    if (pot) {
+        pot[i] += p;
+        pot[j] += p;
+    }
+    f[3 * i] += fx;
+    f[3 * i + 1] += fy;
+    f[3 * i + 2] += fz;
+    f[3 * j] -= fx;
+    f[3 * j + 1] -= fy;
+    f[3 * j + 2] -= fz;
+}
+
+static void zero_out(int n, double *f, double *pot, double *vir) {
+    memset(f, 0, sizeof(double) * 3 * n);
+    if (pot) memset(pot, 0, sizeof(double) * n);
+    if (vir) memset(vir, 0, sizeof(double) * n);
+}
+
+/* TForcePair::ComputeHalf (include/potential/tpair.h:62-71): r < CutRadius
+ * = table max range, fr = -V'(r)/r. */
+int orc_pair_forces(int n, const double *xyz, const int *flag,
+                    const double *lo, const double *L, int pbc, const int *cnt,
+                    const int *code, int maxn, const orc_table *phi, double *f,
+                    double *pot, double *vir) {
+    (void)lo;
+    (void)pbc;
+    zero_out(n, f, pot, vir);
+    double cut = phi->r0 + (double)(phi->n - 1) * phi->dr;
+    for (int i = 0; i < n; i++) {
+        if (!active(flag, i)) continue;
+        for (int k = 0; k < cnt[i]; k++) {
+            int c = code[(long)i * maxn + k], j, s[3];
+            decode(c, &j, s);
+            if (!half_owner(i, j, c % 27) || !active(flag, j)) continue;
+            double d[3];
+            double r2 = pair_d(xyz, L, i, j, s, d);
+            if (r2 < ORC_PROXIMITY) return 3;
+            double r = sqrt(r2);
+            if (r < cut) {
+                double p, dp;
+                int e = orc_spline_read(phi, r, &p, &dp);
+                if (e) return e;
+                return_force(i, j, d, -dp / r, p, f, pot, vir);
+            }
+        }
+    }
+    return 0;
+}
+
+/* EAM, single species: density pass TEmbedding::IterationNode
+ * (team.cpp:81-92, R <= EDENS range), force pass TForceEAM::ComputeHalf
+ * (team.cpp:224-237, R <= PAIR range; F'(rho) by embed.dread per atom),
+ * embedding energy TEmbedding::PostForce (team.cpp:135-145). */
+int orc_eam_forces(int n, const double *xyz, const int *flag, const double *lo,
+                   const double *L, int pbc, const int *cnt, const int *code,
+                   int maxn, const orc_table *phi, const orc_table *edens,
+                   const orc_table *embed, double *f, double *pot, double *vir,
+                   double *rho) {
+    (void)lo;
+    (void)pbc;
+    zero_out(n, f, pot, vir);
+    double *myrho = rho ? rho : (double *)malloc(sizeof(double) * n);
+    double *fp = (double *)malloc(sizeof(double) * n);
+    memset(myrho, 0, sizeof(double) * n);
+    double cut_e = edens->r0 + (double)(edens->n - 1) * edens->dr;
+    double cut_p = phi->r0 + (double)(phi->n - 1) * phi->dr;
+    int err = 0;
+    for (int i = 0; i < n && !err; i++) {
+        if (!active(flag, i)) continue;
+        for (int k = 0; k < cnt[i]; k++) {
+            int c = code[(long)i * maxn + k], j, s[3];
+            decode(c, &j, s);
+            if (!half_owner(i, j, c % 27) || !active(flag, j)) continue;
+            double d[3];
+            double r2 = pair_d(xyz, L, i, j, s, d);
+            if (r2 < ORC_PROXIMITY) {
+                err = 3;
+                break;
+            }
+            double R = sqrt(r2);
+            if (R <= cut_e) {
+                double dens;
+                err = orc_spline_value(edens, R, &dens);
+                if (err) break;
+                myrho[i] += dens;
+                myrho[j] += dens;
+            }
+        }
+    }
+    for (int i = 0; i < n && !err; i++) {
+        double v, dv;
+        if (!active(flag, i)) {
+            fp[i] = 0.0;
+            continue;
+        }
+        err = orc_spline_read(embed, myrho[i], &v, &dv);
+        fp[i] = dv;
+    }
+    for (int i = 0; i < n && !err; i++) {
+        if (!active(flag, i)) continue;
+        for (int k = 0; k < cnt[i]; k++) {
+            int c = code[(long)i * maxn + k], j, s[3];
+            decode(c, &j, s);
+            if (!half_owner(i, j, c % 27) || !active(flag, j)) continue;
+            double d[3];
+            double r2 = pair_d(xyz, L, i, j, s, d);
+            double R = sqrt(r2);
+            if (R <= cut_p) {
+                double p, dp, v, drho;
+                err = orc_spline_read(phi, R, &p, &dp);
+                if (err) break;
+                /* GetRhoDeriv (team.cpp:128-132): 0 beyond the EDENS range */
+                if (R > cut_e)
+                    drho = 0.0;
+                else {
+                    err = orc_spline_read(edens, R, &v, &drho);
+                    if (err) break;
+                }
+                double fr = -(fp[i] * drho + fp[j] * drho + dp) / R;
+                return_force(i, j, d, fr, p, f, pot, vir);
+            }
+        }
+    }
+    for (int i = 0; i < n && !err && pot; i++) {
+        if (!active(flag, i)) continue;
+        double v;
+        err = orc_spline_value(embed, myrho[i], &v);
+        pot[i] += v;
+    }
+    free(fp);
+    if (!rho) free(myrho);
+    return err;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Stillinger-Weber: src/gadget/sw.cpp:62-93 (constants), :118-189 (two-body
+ * with ZBL core and polynomial splice), :191-236 (three-body), :240-273     */
+/* ------------------------------------------------------------------------ */
+int orc_sw_forces(int n, const double *xyz, const int *flag, const double *lo,
+                  const double *L, int pbc, const int *cnt, const int *code,
+                  int maxn, const orc_sw_param *P, double *f, double *pot) {
+    (void)lo;
+    (void)pbc;
+    zero_out(n, f, pot, NULL);
+    double rc = P->a * P->sigma, rc2 = rc * rc;
+    double c1 = P->A * P->eps * P->p * P->B * pow(P->sigma, P->p);
+    double c2 = P->A * P->eps * P->q * pow(P->sigma, P->q);
+    double c3 = P->A * P->eps * P->B * pow(P->sigma, P->p + 1.0);
+    double c4 = P->A * P->eps * pow(P->sigma, P->q + 1.0);
+    double c5 = P->A * P->eps * P->B * pow(P->sigma, P->p);
+    double c6 = P->A * P->eps * pow(P->sigma, P->q);
+    double c7 = P->sigma * P->gamma;
+    double c8 = P->lambda * P->eps;
+    double c9 = 2.0 * P->lambda * P->eps;
+    double c10 = 1.0 / P->icos0;
+    double znor = 0.468521928 / (pow(P->z1, 0.23) + pow(P->z2, 0.23));
+    double zc = 14.39964415 * P->z1 * P->z2;
+    double ze[4] = {3.2000 / znor, 0.9423 / znor, 0.4029 / znor,
+                    0.2016 / znor};
+    double zw[4] = {0.1818, 0.5099, 0.2802, 0.02817};
+    /* two-body over the half list */
+    for (int i = 0; i < n; i++) {
+        if (!active(flag, i)) continue;
+        for (int k = 0; k < cnt[i]; k++) {
+            int c = code[(long)i * maxn + k], j, s[3];
+            decode(c, &j, s);
+            if (!half_owner(i, j, c % 27) || !active(flag, j)) continue;
+            double d[3];
+            double RR = pair_d(xyz, L, i, j, s, d);
+            if (RR < ORC_PROXIMITY) return 3;
+            if (!(RR < rc2)) continue;
+            double r = sqrt(RR), fr, fpot;
+            if (P->usezbl && r < P->sx0) {
+                double ex[4], sp = 0.0, sf = 0.0;
+                for (int m = 0; m < 4; m++) ex[m] = zw[m] * exp(-ze[m] * r);
+                sp = (ex[0] + ex[1] + ex[2] + ex[3]) * zc / r;
+                sf = ((1.0 / r + ze[0]) * ex[0] + (1.0 / r + ze[1]) * ex[1] +
+                      (1.0 / r + ze[2]) * ex[2] + (1.0 / r + ze[3]) * ex[3]) *
+                     zc / (r * r);
+                fr = sf;
+                fpot = 0.5 * sp;
+            } else if (P->usezbl && r < P->sx1) {
+                double g = P->sa1 + P->sa2 * r + P->sa3 * r * r;
+                fpot = 0.5 * (P->sa0 + g * r);
+                fr = -g;
+            } else {
+                double ira = 1.0 / (r - rc);
+                double rp = pow(r, -P->p), rq = pow(r, -P->q);
+                double fex = exp(P->sigma * ira);
+                fr = (c1 * rp - c2 * rq + (c3 * rp - c4 * rq) * ira * ira * r) *
+                     fex / (r * r);
+                fpot = 0.5 * (c5 * rp - c6 * rq) * fex;
+            }
+            f[3 * i] += d[0] * fr;
+            f[3 * i + 1] += d[1] * fr;
+            f[3 * i + 2] += d[2] * fr;
+            f[3 * j] -= d[0] * fr;
+            f[3 * j + 1] -= d[1] * fr;
+            f[3 * j + 2] -= d[2] * fr;
+            if (pot) {
+                pot[i] += fpot;
+                pot[j] += fpot;
+            }
+        }
+    }
+    /* three-body: centre i, entries j (position a) and k (position b > a)
+     * both inside the cut; d1 = x_j - x_i, d2 = x_k - x_i */
+    for (int i = 0; i < n; i++) {
+        if (!active(flag, i)) continue;
+        for (int a = 0; a < cnt[i]; a++) {
+            int ca = code[(long)i * maxn + a], j, sj[3];
+            decode(ca, &j, sj);
+            if (!active(flag, j)) continue;
+            double d1[3];
+            for (int m = 0; m < 3; m++)
+                d1[m] = shifted(xyz[3 * j + m], sj[m], L[m]) - xyz[3 * i + m];
+            double R1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
+            if (!(R1 < rc2)) continue;
+            for (int b = a + 1; b < cnt[i]; b++) {
+                int cb = code[(long)i * maxn + b], kk, sk[3];
+                decode(cb, &kk, sk);
+                if (!active(flag, kk)) continue;
+                double d2[3];
+                for (int m = 0; m < 3; m++)
+                    d2[m] = shifted(xyz[3 * kk + m], sk[m], L[m]) -
+                            xyz[3 * i + m];
+                double R2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+                if (!(R2 < rc2)) continue;
+                double r1 = sqrt(R1), r2 = sqrt(R2);
+                double ira0 = 1.0 / (r1 - rc), ira1 = 1.0 / (r2 - rc);
+                double fi0 = c7 * ira0, fi1 = c7 * ira1;
+                double ir12 = 1.0 / (r1 * r2);
+                double fex = exp(fi0) * exp(fi1);
+                double cs = (d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2]) *
+                            ir12;
+                double dc = cs - c10;
+                double ftheta = c9 * fex * dc;
+                double ft12 = ir12 * ftheta;
+                double fp3 = c8 * fex * dc * dc;
+                double fcos = cs * ftheta;
+                double fr0 = fp3 * fi0 * ira0 / r1, fr1 = fp3 * fi1 * ira1 / r2;
+                double ft0 = fcos / (r1 * r1), ft1 = fcos / (r2 * r2);
+                for (int m = 0; m < 3; m++) {
+                    double g0 = d1[m] * (fr0 + ft0) - d2[m] * ft12;
+                    double g1 = d2[m] * (fr1 + ft1) - d1[m] * ft12;
+                    f[3 * i + m] -= g0 + g1;
+                    f[3 * j + m] += g0;
+                    f[3 * kk + m] += g1;
+                }
+                if (pot) {
+                    double e3 = fp3 * (1. / 3.);
+                    pot[i] += e3;
+                    pot[j] += e3;
+                    pot[kk] += e3;
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Tersoff (Tersoff, PRB 37, 6991 (1988)); cutoff function of the reference  */
+/* include/potential/tersoff.h:247-257 on [R-D, R+D].                        */
+/* E = 1/2 sum_i sum_{j!=i} fc(r_ij) [A e^{-l1 r} - b_ij B e^{-l2 r}]         */
+/* zeta_ij = sum_{k!=i,j} fc(r_ik) g(theta_ijk) exp[(l3 (r_ij - r_ik))^m]     */
+/* g = 1 + c^2/d^2 - c^2/(d^2 + (h - cos)^2), b = (1 + (beta zeta)^n)^(-1/2n) */
+/* ------------------------------------------------------------------------ */
+static inline void ters_fc(const orc_tersoff_param *P, double r, double *fc,
+                           double *dfc) {
+    if (r < P->R - P->D) {
+        *fc = 1.0;
+        *dfc = 0.0;
+    } else if (r < P->R + P->D) {
+        double arg = M_PI * (r - P->R) / (2.0 * P->D);
+        *fc = 0.5 - 0.5 * sin(arg);
+        *dfc = -M_PI / (4.0 * P->D) * cos(arg);
+    } else {
+        *fc = 0.0;
+        *dfc = 0.0;
+    }
+}
+
+int orc_tersoff_forces(int n, const double *xyz, const int *flag,
+                       const double *lo, const double *L, int pbc,
+                       const int *cnt, const int *code, int maxn,
+                       const orc_tersoff_param *P, double *f, double *pot) {
+    (void)lo;
+    (void)pbc;
+    zero_out(n, f, pot, NULL);
+    double rc = P->R + P->D, rc2 = rc * rc;
+    double c2 = P->c * P->c, d2 = P->d * P->d;
+    int m = P->m;
+    double l3m = pow(P->lambda3, (double)m);
+    /* per centre: gather its neighbours inside the cut */
+    int cap = 64;
+    int *nj = (int *)malloc(sizeof(int) * cap);
+    double *dv = (double *)malloc(sizeof(double) * 3 * cap);
+    double *rr = (double *)malloc(sizeof(double) * cap);
+    for (int i = 0; i < n; i++) {
+        if (!active(flag, i)) continue;
+        int nn = 0;
+        for (int a = 0; a < cnt[i]; a++) {
+            int c = code[(long)i * maxn + a], j, s[3];
+            decode(c, &j, s);
+            if (!active(flag, j)) continue;
+            double d[3];
+            for (int q = 0; q < 3; q++)
+                d[q] = shifted(xyz[3 * j + q], s[q], L[q]) - xyz[3 * i + q];
+            double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (r2 < ORC_PROXIMITY) {
+                free(nj);
+                free(dv);
+                free(rr);
+                return 3;
+            }
+            if (!(r2 < rc2)) continue;
+            if (nn == cap) {
+                cap *= 2;
+                nj = (int *)realloc(nj, sizeof(int) * cap);
+                dv = (double *)realloc(dv, sizeof(double) * 3 * cap);
+                rr = (double *)realloc(rr, sizeof(double) * cap);
+            }
+            nj[nn] = j;
+            dv[3 * nn] = d[0];
+            dv[3 * nn + 1] = d[1];
+            dv[3 * nn + 2] = d[2];
+            rr[nn] = sqrt(r2);
+            nn++;
+        }
+        for (int a = 0; a < nn; a++) {
+            double rij = rr[a];
+            const double *dij = dv + 3 * a;
+            double fcij, dfcij;
+            ters_fc(P, rij, &fcij, &dfcij);
+            /* zeta and its derivative pieces */
+            double zeta = 0.0;
+            for (int b = 0; b < nn; b++) {
+                if (b == a) continue;
+                double rik = rr[b];
+                const double *dik = dv + 3 * b;
+                double fck, dfck;
+                ters_fc(P, rik, &fck, &dfck);
+                double cs = (dij[0] * dik[0] + dij[1] * dik[1] +
+                             dij[2] * dik[2]) /
+                            (rij * rik);
+                double hc = P->h - cs;
+                double g = 1.0 + c2 / d2 - c2 / (d2 + hc * hc);
+                double ex = exp(l3m * pow(rij - rik, (double)m));
+                zeta += fck * g * ex;
+            }
+            double fR = P->A * exp(-P->lambda1 * rij);
+            double fA = -P->B * exp(-P->lambda2 * rij);
+            double bz = 1.0, dbz = 0.0;
+            if (zeta > 0.0) {
+                double bzn = pow(P->beta * zeta, P->n);
+                bz = pow(1.0 + bzn, -1.0 / (2.0 * P->n));
+                dbz = -0.5 * pow(1.0 + bzn, -1.0 / (2.0 * P->n) - 1.0) *
+                      pow(P->beta, P->n) * pow(zeta, P->n - 1.0);
+            }
+            double Vij = fcij * (fR + bz * fA);
+            if (pot) {
+                pot[i] += 0.5 * Vij;
+            }
+            /* pair part: dV/drij at fixed zeta */
+            double dVdr = dfcij * (fR + bz * fA) +
+                          fcij * (-P->lambda1 * fR + bz * (-P->lambda2) * fA);
+            /* E contains 1/2 V_ij; gradient wrt x_j of r_ij is dij/rij */
+            double pref = 0.5 * dVdr / rij;
+            for (int q = 0; q < 3; q++) {
+                double g = pref * dij[q];
+                f[3 * i + q] += g;
+                f[3 * nj[a] + q] -= g;
+            }
+            if (zeta > 0.0) {
+                /* dE/dzeta = 1/2 fc_ij fA db/dzeta */
+                double dEz = 0.5 * fcij * fA * dbz;
+                for (int b = 0; b < nn; b++) {
+                    if (b == a) continue;
+                    double rik = rr[b];
+                    const double *dik = dv + 3 * b;
+                    double fck, dfck;
+                    ters_fc(P, rik, &fck, &dfck);
+                    double rr2 = rij * rik;
+                    double cs = (dij[0] * dik[0] + dij[1] * dik[1] +
+                                 dij[2] * dik[2]) /
+                                rr2;
+                    double hc = P->h - cs;
+                    double den = d2 + hc * hc;
+                    double g = 1.0 + c2 / d2 - c2 / den;
+                    double dg_dc = -2.0 * c2 * hc / (den * den);
+                    double dlt = rij - rik;
+                    double ex = exp(l3m * pow(dlt, (double)m));
+                    double dex = ex * l3m * (double)m * pow(dlt, (double)(m - 1));
+                    /* zeta term T = fck g ex; gradients w.r.t. x_j and x_k
+                     * (x_i gets minus the sum) */
+                    double gj[3], gk[3];
+                    for (int q = 0; q < 3; q++) {
+                        double uj = dij[q] / rij, uk = dik[q] / rik;
+                        double dcj = (uk - cs * uj) / rij;
+                        double dck = (uj - cs * uk) / rik;
+                        gj[q] = fck * dg_dc * dcj * ex + fck * g * dex * uj;
+                        gk[q] = dfck * uk * g * ex + fck * dg_dc * dck * ex -
+                                fck * g * dex * uk;
+                    }
+                    for (int q = 0; q < 3; q++) {
+                        f[3 * nj[a] + q] -= dEz * gj[q];
+                        f[3 * nj[b] + q] -= dEz * gk[q];
+                        f[3 * i + q] += dEz * (gj[q] + gk[q]);
+                    }
+                }
+            }
+        }
+    }
+    free(nj);
+    free(dv);
+    free(rr);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* velocity Verlet: src/omd/integrator.cpp:225-259                           */
+/* ------------------------------------------------------------------------ */
+void orc_vv_kick_drift(int n, double *xyz, double *vel, const double *f,
+                       const double *mass, const int *flag, double dt) {
+    double hdt = 0.5 * dt;
+    for (int i = 0; i < n; i++) {
+        if (!active(flag, i)) continue;
+        if (flag && (flag[i] & 8)) continue; /* fixed atom */
+        double M = mass[i];
+        for (int d = 0; d < 3; d++) vel[3 * i + d] += hdt * f[3 * i + d] / M;
+        for (int d = 0; d < 3; d++) xyz[3 * i + d] += dt * (vel[3 * i + d]);
+    }
+}
+
+void orc_vv_kick(int n, double *vel, const double *f, const double *mass,
+                 const int *flag, double dt) {
+    double hdt = 0.5 * dt;
+    for (int i = 0; i < n; i++) {
+        if (!active(flag, i)) continue;
+        if (flag && (flag[i] & 8)) continue;
+        double M = mass[i];
+        for (int d = 0; d < 3; d++) vel[3 * i + d] += hdt * f[3 * i + d] / M;
+    }
+}
+
+/* MDSystem::MeasureKinetic (system.cpp:1077-1094) */
+double orc_kinetic(int n, const double *vel, const double *mass,
+                   const int *flag, double *vmax2) {
+    double acc = 0.0, vm = 0.0;
+    for (int i = 0; i < n; i++) {
+        if (!active(flag, i)) continue;
+        const double *v = vel + 3 * i;
+        double s = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        acc += mass[i] * s;
+        if (s > vm) vm = s;
+    }
+    if (vmax2) *vmax2 = vm;
+    return 0.5 * acc;
+}
+
+/* Re-ownership of atoms that left the periodic box: the image inside the
+ * box, x -+ L, becomes the atom (systemgrid.cpp:479-506 with the shift of
+ * comhandler.cpp:288-355). */
+void orc_wrap(int n, double *xyz, const double *lo, const double *L, int pbc) {
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) {
+            if (!((pbc >> d) & 1)) continue;
+            double *x = xyz + 3 * i + d;
+            if (*x < lo[d])
+                *x = *x + L[d];
+            else if (*x >= lo[d] + L[d])
+                *x = *x + (-L[d]);
+        }
+}
+
+static int eval_forces(int kind, int n, const double *xyz, const int *flag,
+                       const double *lo, const double *L, int pbc,
+                       const int *cnt, const int *code, int maxn,
+                       const orc_table *t0, const orc_table *t1,
+                       const orc_table *t2, double *f, double *pot,
+                       double *vir) {
+    if (kind == 0)
+        return orc_pair_forces(n, xyz, flag, lo, L, pbc, cnt, code, maxn, t0, f,
+                               pot, vir);
+    return orc_eam_forces(n, xyz, flag, lo, L, pbc, cnt, code, maxn, t0, t1, t2,
+                          f, pot, vir, NULL);
+}
+
+static double sum_active(int n, const double *a, const int *flag) {
+    double s = 0.0;
+    for (int i = 0; i < n; i++)
+        if (active(flag, i)) s += a[i];
+    return s;
+}
+
+int orc_run(int kind, int n, double *xyz, double *vel, const int *flag,
+            const double *mass, const double *lo, const double *L, int pbc,
+            const orc_table *t0, const orc_table *t1, const orc_table *t2,
+            double cut, double skin, int refresh, int maxn, int nsteps,
+            double dt, double *energies) {
+    int *cnt = (int *)malloc(sizeof(int) * n);
+    int *code = (int *)malloc(sizeof(int) * (long)n * maxn);
+    double *f = (double *)malloc(sizeof(double) * 3 * n);
+    double *pot = (double *)malloc(sizeof(double) * n);
+    double *vir = (double *)malloc(sizeof(double) * n);
+    double rl = cut + skin;
+    int err = 0;
+    if (refresh < 1) refresh = 1;
+    /* FirstRun: one force evaluation, BasePotential, then f = 0 */
+    orc_wrap(n, xyz, lo, L, pbc);
+    if (orc_nbr_full(n, xyz, flag, lo, L, pbc, rl, maxn, cnt, code)) {
+        err = -1;
+        goto out;
+    }
+    err = eval_forces(kind, n, xyz, flag, lo, L, pbc, cnt, code, maxn, t0, t1,
+                      t2, f, pot, vir);
+    if (err) goto out;
+    double base = sum_active(n, pot, flag);
+    memset(f, 0, sizeof(double) * 3 * n);
+    for (int s = 0; s < nsteps; s++) {
+        orc_vv_kick_drift(n, xyz, vel, f, mass, flag, dt);
+        if ((s + 1) % refresh == 0) {
+            orc_wrap(n, xyz, lo, L, pbc);
+            if (orc_nbr_full(n, xyz, flag, lo, L, pbc, rl, maxn, cnt, code)) {
+                err = -1;
+                goto out;
+            }
+        }
+        err = eval_forces(kind, n, xyz, flag, lo, L, pbc, cnt, code, maxn, t0,
+                          t1, t2, f, pot, vir);
+        if (err) goto out;
+        orc_vv_kick(n, vel, f, mass, flag, dt);
+        if (energies) {
+            double ek = orc_kinetic(n, vel, mass, flag, NULL);
+            double ep = sum_active(n, pot, flag);
+            energies[3 * s] = ek;
+            energies[3 * s + 1] = ep;
+            energies[3 * s + 2] = ek + ep - base;
+        }
+    }
+out:
+    free(cnt);
+    free(code);
+    free(f);
+    free(pot);
+    free(vir);
+    return err;
+}

@@ -1,0 +1,113 @@
+/*
+ * FP64 CPU oracle for the Object-MD force/integration path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this code; it is used by tests/, by the smoke check and by the CPU
+ * baseline leg of the benchmark, always as the checker and never as the
+ * thing measured or shipped.
+ *
+ * Every function restates the reference algorithm in plain C (FP64,
+ * no FMA contraction) and cites the reference file:line it follows.
+ * Which checks pin each function is listed in the header of omd_oracle.c.
+ */
+#ifndef OMD_ORACLE_H
+#define OMD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Cubic-spline table, coefficients interleaved {a,b,c,d} per knot
+ * (treader.h:29 CoefStruct).  n = NumberOfData, dr = Spacing, r0 = Offset. */
+typedef struct {
+    int n;
+    int rmult;          /* 1: table holds r*V(r) (Format RMULT) */
+    double dr;
+    double r0;
+    const double *abcd; /* 4*n doubles */
+} orc_table;
+
+/* --- spline (src/omd/treader.cpp) --------------------------------------- */
+void orc_spline_coef(int n, double dr, const double *y, double *abcd);
+int orc_spline_read(const orc_table *t, double r, double *val, double *dval);
+int orc_spline_value(const orc_table *t, double r, double *val);
+int orc_table_load(const char *path, const char *name, int maxn, double *y,
+                   int *n, double *dr, double *r0, int *rmult);
+
+/* --- lattices (src/gadget/FCC.cpp, include/crystal/Diamond.h,
+ *     include/crystal/Crystalline.h) ------------------------------------- */
+int orc_fcc100(int xml, int yml, int zml, double a, double *xyz, double *box6);
+int orc_diamond100(int xml, int yml, int zml, double a, double *xyz,
+                   double *box6);
+
+/* --- neighbour lists (src/gadget/VerletList*.cpp membership) ------------ */
+/* Full list: for each atom the sorted codes j*27 + img, img = (sx+1) +
+ * 3(sy+1) + 9(sz+1), of every periodic image of every active atom j with
+ * r^2 <= rlist^2 (shift-then-subtract FP64).  Returns 0, or -1 if some atom
+ * has more than maxn entries (cnt still holds the true counts). */
+int orc_nbr_full(int n, const double *xyz, const int *flag, const double *lo,
+                 const double *L, int pbc, double rlist, int maxn, int *cnt,
+                 int *code);
+int orc_nbr_brute(int n, const double *xyz, const int *flag, const double *lo,
+                  const double *L, int pbc, double rlist, int maxn, int *cnt,
+                  int *code);
+
+/* --- forces --------------------------------------------------------------*/
+/* Outputs are overwritten.  Return 0 on success, nonzero error codes:
+ * 1 table upper range exceeded, 2 table lower range exceeded,
+ * 3 overlapping atoms (r^2 < 1e-6). */
+int orc_pair_forces(int n, const double *xyz, const int *flag,
+                    const double *lo, const double *L, int pbc, const int *cnt,
+                    const int *code, int maxn, const orc_table *phi,
+                    double *f, double *pot, double *vir);
+int orc_eam_forces(int n, const double *xyz, const int *flag, const double *lo,
+                   const double *L, int pbc, const int *cnt, const int *code,
+                   int maxn, const orc_table *phi, const orc_table *edens,
+                   const orc_table *embed, double *f, double *pot, double *vir,
+                   double *rho);
+
+/* Stillinger-Weber parameter block (tables/sw.silicon keys, sw.cpp:35-93). */
+typedef struct {
+    double eps, A, B, p, q, sigma, a, lambda, gamma, icos0;
+    int usezbl;
+    double sx0, sx1, sa0, sa1, sa2, sa3;
+    double z1, z2; /* atomic numbers for the ZBL core */
+} orc_sw_param;
+int orc_sw_forces(int n, const double *xyz, const int *flag, const double *lo,
+                  const double *L, int pbc, const int *cnt, const int *code,
+                  int maxn, const orc_sw_param *p, double *f, double *pot);
+
+/* Tersoff (PRB 37 6991 / PRB 38 9902 form, single species). */
+typedef struct {
+    double A, B, lambda1, lambda2, lambda3, alpha, beta, n, c, d, h, R, D;
+    int m; /* exponent of lambda3 term: 3 (Tersoff) */
+} orc_tersoff_param;
+int orc_tersoff_forces(int n, const double *xyz, const int *flag,
+                       const double *lo, const double *L, int pbc,
+                       const int *cnt, const int *code, int maxn,
+                       const orc_tersoff_param *p, double *f, double *pot);
+
+/* --- velocity Verlet (src/omd/integrator.cpp:225-259) ------------------- */
+void orc_vv_kick_drift(int n, double *xyz, double *vel, const double *f,
+                       const double *mass, const int *flag, double dt);
+void orc_vv_kick(int n, double *vel, const double *f, const double *mass,
+                 const int *flag, double dt);
+double orc_kinetic(int n, const double *vel, const double *mass,
+                   const int *flag, double *vmax2);
+void orc_wrap(int n, double *xyz, const double *lo, const double *L, int pbc);
+
+/* Whole run driver: FirstRun (system.cpp:860-873) then nsteps of
+ * RunKernel/Integrate (system.cpp:964-983), neighbour list rebuilt every
+ * `refresh` steps with rlist = cut + skin.  kind 0 = pair table, 1 = EAM.
+ * energies[3*s+{0,1,2}] = Ek, Ep, Ek+Ep-BasePotential after step s
+ * (may be NULL).  Returns 0 or an error code as above, -1 list overflow. */
+int orc_run(int kind, int n, double *xyz, double *vel, const int *flag,
+            const double *mass, const double *lo, const double *L, int pbc,
+            const orc_table *t0, const orc_table *t1, const orc_table *t2,
+            double cut, double skin, int refresh, int maxn, int nsteps,
+            double dt, double *energies);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

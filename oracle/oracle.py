@@ -1,0 +1,262 @@
+"""ctypes access to the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, always as the checker.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+SRC = os.path.join(HERE, "omd_oracle.c")
+
+_lib = None
+
+
+def build(force=False):
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
+        tmp = LIB + ".tmp"
+        subprocess.run(["gcc", "-O2", "-std=c99", "-D_GNU_SOURCE", "-Wall",
+                        "-ffp-contract=off", "-fPIC", "-shared", "-o", tmp, SRC, "-lm"],
+                       check=True)
+        os.replace(tmp, LIB)
+    return LIB
+
+
+class Table(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("rmult", ctypes.c_int), ("dr", ctypes.c_double),
+                ("r0", ctypes.c_double), ("abcd", ctypes.POINTER(ctypes.c_double))]
+
+
+class SwParam(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in
+                ("eps", "A", "B", "p", "q", "sigma", "a", "lambda_", "gamma", "icos0")] + \
+               [("usezbl", ctypes.c_int)] + \
+               [(k, ctypes.c_double) for k in
+                ("sx0", "sx1", "sa0", "sa1", "sa2", "sa3", "z1", "z2")]
+
+
+class TersoffParam(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in
+                ("A", "B", "lambda1", "lambda2", "lambda3", "alpha", "beta", "n", "c",
+                 "d", "h", "R", "D")] + [("m", ctypes.c_int)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        _lib = ctypes.CDLL(LIB)
+    return _lib
+
+
+D = ctypes.POINTER(ctypes.c_double)
+I = ctypes.POINTER(ctypes.c_int)
+
+
+def _d(a):
+    return a.ctypes.data_as(D) if a is not None else None
+
+
+def _i(a):
+    return a.ctypes.data_as(I) if a is not None else None
+
+
+class Spline:
+    """Oracle spline table; keeps the coefficient array alive."""
+
+    def __init__(self, y, dr, r0, rmult=False):
+        y = np.ascontiguousarray(y, np.float64)
+        self.abcd = np.zeros(4 * len(y))
+        lib().orc_spline_coef(ctypes.c_int(len(y)), ctypes.c_double(dr), _d(y), _d(self.abcd))
+        self.t = Table(len(y), int(bool(rmult)), float(dr), float(r0), _d(self.abcd))
+
+    @classmethod
+    def from_table(cls, t):
+        return cls(t.y, t.dr, t.r0, t.rmult)
+
+    def read(self, r):
+        v = ctypes.c_double()
+        dv = ctypes.c_double()
+        rc = lib().orc_spline_read(ctypes.byref(self.t), ctypes.c_double(r), ctypes.byref(v),
+                                   ctypes.byref(dv))
+        return rc, v.value, dv.value
+
+    def value(self, r):
+        v = ctypes.c_double()
+        rc = lib().orc_spline_value(ctypes.byref(self.t), ctypes.c_double(r), ctypes.byref(v))
+        return rc, v.value
+
+    @property
+    def coef(self):
+        return self.abcd.reshape(-1, 4)
+
+
+def table_load(path, name, maxn=100000):
+    y = np.zeros(maxn)
+    n = ctypes.c_int()
+    dr = ctypes.c_double()
+    r0 = ctypes.c_double()
+    rm = ctypes.c_int()
+    rc = lib().orc_table_load(path.encode(), name.encode(), maxn, _d(y), ctypes.byref(n),
+                              ctypes.byref(dr), ctypes.byref(r0), ctypes.byref(rm))
+    if rc:
+        raise RuntimeError(f"orc_table_load({path},{name}) = {rc}")
+    return y[:n.value].copy(), dr.value, r0.value, bool(rm.value)
+
+
+def fcc100(xml, yml, zml, a):
+    n = lib().orc_fcc100(xml, yml, zml, ctypes.c_double(a), None, None)
+    xyz = np.zeros((n, 3))
+    box = np.zeros(6)
+    lib().orc_fcc100(xml, yml, zml, ctypes.c_double(a), _d(xyz), _d(box))
+    return xyz, box
+
+
+def diamond100(xml, yml, zml, a):
+    n = lib().orc_diamond100(xml, yml, zml, ctypes.c_double(a), None, None)
+    xyz = np.zeros((n, 3))
+    box = np.zeros(6)
+    lib().orc_diamond100(xml, yml, zml, ctypes.c_double(a), _d(xyz), _d(box))
+    return xyz, box
+
+
+def _geom(x, lo, L):
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    lo = np.ascontiguousarray(lo, np.float64)
+    L = np.ascontiguousarray(L, np.float64)
+    return x, lo, L
+
+
+def neighbors(x, lo, L, pbc, rlist, maxn=128, flag=None, brute=False):
+    """Full list, per atom sorted codes j*27 + img."""
+    x, lo, L = _geom(x, lo, L)
+    n = len(x)
+    cnt = np.zeros(n, np.int32)
+    code = np.zeros((n, maxn), np.int32)
+    fl = None if flag is None else np.ascontiguousarray(flag, np.int32)
+    fn = lib().orc_nbr_brute if brute else lib().orc_nbr_full
+    rc = fn(n, _d(x), _i(fl), _d(lo), _d(L), pbc, ctypes.c_double(rlist), maxn, _i(cnt),
+            _i(code))
+    if rc:
+        raise RuntimeError(f"oracle neighbour list failed: {rc}")
+    return cnt, code
+
+
+def wrap(x, lo, L, pbc):
+    x = np.ascontiguousarray(x, np.float64).copy()
+    lo = np.ascontiguousarray(lo, np.float64)
+    L = np.ascontiguousarray(L, np.float64)
+    lib().orc_wrap(len(x), _d(x), _d(lo), _d(L), pbc)
+    return x
+
+
+def pair_forces(x, lo, L, pbc, nbr, phi, flag=None):
+    x, lo, L = _geom(x, lo, L)
+    cnt, code = nbr
+    n = len(x)
+    f = np.zeros((n, 3))
+    pot = np.zeros(n)
+    vir = np.zeros(n)
+    fl = None if flag is None else np.ascontiguousarray(flag, np.int32)
+    rc = lib().orc_pair_forces(n, _d(x), _i(fl), _d(lo), _d(L), pbc, _i(cnt), _i(code),
+                               code.shape[1], ctypes.byref(phi.t), _d(f), _d(pot), _d(vir))
+    if rc:
+        raise RuntimeError(f"orc_pair_forces = {rc}")
+    return f, pot, vir
+
+
+def eam_forces(x, lo, L, pbc, nbr, phi, edens, embed, flag=None):
+    x, lo, L = _geom(x, lo, L)
+    cnt, code = nbr
+    n = len(x)
+    f = np.zeros((n, 3))
+    pot = np.zeros(n)
+    vir = np.zeros(n)
+    rho = np.zeros(n)
+    fl = None if flag is None else np.ascontiguousarray(flag, np.int32)
+    rc = lib().orc_eam_forces(n, _d(x), _i(fl), _d(lo), _d(L), pbc, _i(cnt), _i(code),
+                              code.shape[1], ctypes.byref(phi.t), ctypes.byref(edens.t),
+                              ctypes.byref(embed.t), _d(f), _d(pot), _d(vir), _d(rho))
+    if rc:
+        raise RuntimeError(f"orc_eam_forces = {rc}")
+    return f, pot, vir, rho
+
+
+def sw_forces(x, lo, L, pbc, nbr, p, flag=None):
+    x, lo, L = _geom(x, lo, L)
+    cnt, code = nbr
+    n = len(x)
+    f = np.zeros((n, 3))
+    pot = np.zeros(n)
+    fl = None if flag is None else np.ascontiguousarray(flag, np.int32)
+    rc = lib().orc_sw_forces(n, _d(x), _i(fl), _d(lo), _d(L), pbc, _i(cnt), _i(code),
+                             code.shape[1], ctypes.byref(p), _d(f), _d(pot))
+    if rc:
+        raise RuntimeError(f"orc_sw_forces = {rc}")
+    return f, pot
+
+
+def tersoff_forces(x, lo, L, pbc, nbr, p, flag=None):
+    x, lo, L = _geom(x, lo, L)
+    cnt, code = nbr
+    n = len(x)
+    f = np.zeros((n, 3))
+    pot = np.zeros(n)
+    fl = None if flag is None else np.ascontiguousarray(flag, np.int32)
+    rc = lib().orc_tersoff_forces(n, _d(x), _i(fl), _d(lo), _d(L), pbc, _i(cnt), _i(code),
+                                  code.shape[1], ctypes.byref(p), _d(f), _d(pot))
+    if rc:
+        raise RuntimeError(f"orc_tersoff_forces = {rc}")
+    return f, pot
+
+
+def run(kind, x, v, mass, lo, L, pbc, tables, cut, skin, refresh, nsteps, dt, maxn=128,
+        flag=None):
+    """Velocity-Verlet run; x, v updated copies returned with energies (nsteps,3)."""
+    x, lo, L = _geom(x, lo, L)
+    x = x.copy()
+    v = np.ascontiguousarray(v, np.float64).reshape(-1, 3).copy()
+    n = len(x)
+    m = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, np.float64), (n,)))
+    en = np.zeros((max(nsteps, 1), 3))
+    t = [tb.t if tb is not None else None for tb in (list(tables) + [None] * 3)[:3]]
+    fl = None if flag is None else np.ascontiguousarray(flag, np.int32)
+    rc = lib().orc_run(kind, n, _d(x), _d(v), _i(fl), _d(m), _d(lo), _d(L), pbc,
+                       ctypes.byref(t[0]), ctypes.byref(t[1]) if t[1] else None,
+                       ctypes.byref(t[2]) if t[2] else None, ctypes.c_double(cut),
+                       ctypes.c_double(skin), refresh, maxn, nsteps, ctypes.c_double(dt),
+                       _d(en))
+    if rc:
+        raise RuntimeError(f"orc_run = {rc}")
+    return x, v, en[:nsteps]
+
+
+def vv_kick_drift(x, v, f, mass, dt, flag=None):
+    """In place: v += (dt/2 f)/m; x += dt v."""
+    n = len(x)
+    m = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, np.float64), (n,)))
+    fl = None if flag is None else np.ascontiguousarray(flag, np.int32)
+    lib().orc_vv_kick_drift(n, _d(x), _d(v), _d(np.ascontiguousarray(f)), _d(m), _i(fl),
+                            ctypes.c_double(dt))
+
+
+def vv_kick(v, f, mass, dt, flag=None):
+    n = len(v)
+    m = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, np.float64), (n,)))
+    fl = None if flag is None else np.ascontiguousarray(flag, np.int32)
+    lib().orc_vv_kick(n, _d(v), _d(np.ascontiguousarray(f)), _d(m), _i(fl),
+                      ctypes.c_double(dt))
+
+
+def _declare():
+    L = lib()
+    L.orc_spline_read.argtypes = [ctypes.POINTER(Table), ctypes.c_double, D, D]
+    L.orc_spline_value.argtypes = [ctypes.POINTER(Table), ctypes.c_double, D]
+    L.orc_kinetic.restype = ctypes.c_double
+
+
+_declare()

@@ -1,0 +1,110 @@
+"""Shared builders of the test systems (synthetic inputs of the benchmark
+configurations at reduced size)."""
+import functools
+import os
+
+import numpy as np
+
+from objectmd_amd import lattice, tables
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@functools.lru_cache(maxsize=None)
+def johnson_al():
+    return tables.JohnsonEAM()
+
+
+@functools.lru_cache(maxsize=None)
+def eam_tables(n=5000):
+    return johnson_al().tables(n)
+
+
+def wrap_np(x, lo, L, pbc):
+    """Reference image arithmetic: x + L below the box, x + (-L) above."""
+    x = x.copy()
+    for d in range(3):
+        if (pbc >> d) & 1:
+            lo_d, L_d = lo[d], L[d]
+            below = x[:, d] < lo_d
+            x[below, d] = x[below, d] + L_d
+            above = x[:, d] >= lo_d + L_d
+            x[above, d] = x[above, d] + (-L_d)
+    return x
+
+
+def eam_system(cells=6, seed=12346, disp=0.02, temperature=300.0, a=4.05):
+    """fcc Al, PBC xyz, Gaussian displacements and MB velocities."""
+    pos, (lo, hi) = lattice.fcc100(cells, cells, cells, a)
+    rng = np.random.default_rng(seed)
+    x = pos + rng.normal(0.0, disp, pos.shape)
+    L = hi - lo
+    x = wrap_np(x, lo, L, 7)
+    m = tables.mass_sim(tables.MASS_AL)
+    v = lattice.maxwell_boltzmann(len(x), m, temperature, seed + 1)
+    return dict(x=x, v=v, lo=lo, hi=hi, L=L, pbc=7, mass=m)
+
+
+def lj_system(cells=10, seed=12345, temperature=40.0, a=5.26):
+    pos, (lo, hi) = lattice.fcc100(cells, cells, cells, a)
+    m = tables.mass_sim(tables.MASS_AR)
+    v = lattice.maxwell_boltzmann(len(pos), m, temperature, seed)
+    L = hi - lo
+    return dict(x=pos.copy(), v=v, lo=lo, hi=hi, L=L, pbc=7, mass=m)
+
+
+def rel_rms(a, b):
+    a = np.asarray(a).reshape(-1)
+    b = np.asarray(b).reshape(-1)
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+def read_xyz(path):
+    """Positions of a reference fixture (plain rows or '#$ Fields' rows)."""
+    rows = []
+    with open(path) as fl:
+        for line in fl:
+            line = line.strip()
+            if not line or line.startswith("#"):
+                continue
+            rows.append([float(t) for t in line.split()[:3]])
+    return np.array(rows)
+
+
+SW_SILICON = os.path.join(GOLDEN, "ref", "sw.silicon")
+TERSOFF_SILICON = os.path.join(GOLDEN, "ref", "tersoff.silicon")
+
+
+def read_params(path):
+    """key value... lines of a reference parameter file."""
+    out = {}
+    with open(path) as fl:
+        for line in fl:
+            t = line.split()
+            if not t or t[0].startswith("#"):
+                continue
+            out[t[0]] = t[1:]
+    return out
+
+
+def sw_params(z1=14.0, z2=14.0):
+    """19 values in the order of omdg_set_sw from tables/sw.silicon."""
+    p = read_params(SW_SILICON)
+    g = lambda k, i=0: float(p[k][i])
+    usezbl = p.get("zbl", ["no"])[0] in ("yes", "true", "1")
+    return [g("sw.epsilon"), g("sw.A"), g("sw.B"), g("sw.p"), g("sw.q"), g("sw.sigma"),
+            g("sw.a"), g("sw.lambda"), g("sw.gamma"), g("sw.invers_cos0"),
+            1.0 if usezbl else 0.0, g("spline.range", 0), g("spline.range", 1),
+            g("spline.coeff", 0), g("spline.coeff", 1), g("spline.coeff", 2),
+            g("spline.coeff", 3), z1, z2]
+
+
+def tersoff_params(R=None, D=None):
+    """13 values in the order of omdg_set_tersoff from tables/tersoff.silicon
+    (Si(B), PRB 37 6991); R and D may be overridden."""
+    p = read_params(TERSOFF_SILICON)
+    g = lambda k: float(p[k][0])
+    return [g("tersoff.A"), g("tersoff.B"), g("tersoff.lambda_1"), g("tersoff.lambda_2"),
+            g("tersoff.lambda_3"), g("tersoff.alpha"), g("tersoff.beta"), g("tersoff.n"),
+            g("tersoff.c"), g("tersoff.d"), g("tersoff.h"),
+            g("tersoff.R") if R is None else R, g("tersoff.D") if D is None else D]

@@ -1,0 +1,282 @@
+"""GPU parity: the HIP engine through the C ABI against the FP64 oracle.
+
+Bars: neighbour lists bit-exact after per-atom sort; FP32-force paths within
+1e-5 relative RMS force error; FP64 pair path within 1e-12."""
+import sys
+
+import numpy as np
+import pytest
+
+import objectmd_amd as om
+from objectmd_amd import lattice, tables
+from oracle import oracle as O
+from tests.common import eam_system, eam_tables, lj_system, rel_rms, wrap_np
+
+pytestmark = pytest.mark.gpu
+
+FORCE_RTOL = 1e-5   # relative RMS force error of the FP32 force paths
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _eam_engine(s, skin=0.3):
+    eng = om.Engine()
+    eng.set_box(s["lo"], s["hi"], s["pbc"])
+    eng.upload_atoms(s["x"], s["v"], mass=[s["mass"]])
+    t = eam_tables()
+    eng.add_table(om.TAB_PAIR, 0, 0, t["PAIR"].y, t["PAIR"].dr, t["PAIR"].r0)
+    eng.add_table(om.TAB_EDENS, 0, 0, t["EDENS"].y, t["EDENS"].dr, t["EDENS"].r0)
+    eng.add_table(om.TAB_EMBED, 0, 0, t["EMBED"].y, t["EMBED"].dr, t["EMBED"].r0)
+    eng.set_potential(om.POT_EAM)
+    eng.set_neighbor(skin, 10, 0)
+    return eng
+
+
+def _oracle_eam(x, s, rlist):
+    t = eam_tables()
+    sp = [O.Spline.from_table(t[k]) for k in ("PAIR", "EDENS", "EMBED")]
+    nb = O.neighbors(x, s["lo"], s["L"], s["pbc"], rlist, maxn=96)
+    return O.eam_forces(x, s["lo"], s["L"], s["pbc"], nb, *sp)
+
+
+def _lj_engine(s, fp64=True, skin=0.3):
+    eng = om.Engine()
+    eng.set_box(s["lo"], s["hi"], s["pbc"])
+    eng.upload_atoms(s["x"], s["v"], mass=[s["mass"]])
+    t = tables.lj_table()
+    eng.add_table(om.TAB_PAIR, 0, 0, t.y, t.dr, t.r0, t.rmult)
+    eng.set_potential(om.POT_PAIR, fp64=fp64)
+    eng.set_neighbor(skin, 10, 0)
+    return eng
+
+
+# ---------------------------------------------------------------------------
+# neighbour lists
+# ---------------------------------------------------------------------------
+def _list_case(name):
+    rng = np.random.default_rng(hash(name) % 1000)
+    if name == "lj_fcc":
+        s = lj_system(cells=10)
+        x = wrap_np(s["x"] + rng.normal(0, 0.1, s["x"].shape), s["lo"], s["L"], 7)
+        return x, s["lo"], s["hi"], 7, 8.5 + 0.3
+    if name == "eam_fcc":
+        s = eam_system(cells=8, disp=0.05)
+        return s["x"], s["lo"], s["hi"], 7, 5.5 + 0.3
+    if name in ("gas_pbc", "gas_xy", "gas_free"):
+        lo = np.array([-2.0, 1.0, 0.5])
+        hi = lo + np.array([23.0, 19.5, 21.25])
+        x = lo + rng.random((3000, 3)) * (hi - lo)
+        pbc = {"gas_pbc": 7, "gas_xy": 3, "gas_free": 0}[name]
+        return x, lo, hi, pbc, 3.7
+    if name == "small_box":
+        lo = np.zeros(3)
+        hi = np.array([7.0, 7.5, 8.0])
+        x = lo + rng.random((60, 3)) * (hi - lo)
+        return x, lo, hi, 7, 3.6
+    if name == "slab_xy":
+        pos, (lo, hi) = lattice.diamond100(6, 6, 4, 5.431)
+        x = wrap_np(pos + rng.normal(0, 0.1, pos.shape), lo, hi - lo, 3)
+        return x, lo, hi, 3, 3.2
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", ["lj_fcc", "eam_fcc", "gas_pbc", "gas_xy", "gas_free",
+                                  "small_box", "slab_xy"])
+def test_neighbor_list_bit_exact(name):
+    x, lo, hi, pbc, rlist = _list_case(name)
+    say(f"[neighbor] case={name} n={len(x)} pbc={pbc} rlist={rlist}")
+    cnt_o, code_o = O.neighbors(x, lo, hi - lo, pbc, rlist, maxn=160)
+    eng = om.Engine()
+    eng.set_box(lo, hi, pbc)
+    eng.upload_atoms(x, mass=[1.0])
+    # a pair table whose max range is the list radius minus the skin
+    cut = rlist - 0.25
+    r = np.linspace(0.5, cut, 200)
+    eng.add_table(om.TAB_PAIR, 0, 0, np.zeros_like(r), r[1] - r[0], 0.5)
+    eng.set_potential(om.POT_PAIR, fp64=True)
+    eng.set_neighbor(rlist - (0.5 + (r[1] - r[0]) * 199), 10, 0)
+    eng.neighbor_build()
+    cnt_g, code_g = eng.neighbor_export(maxn=160)
+    # the engine list radius is cut + skin computed in FP64 like the reference
+    rl_engine = (0.5 + (r[1] - r[0]) * 199) + (rlist - (0.5 + (r[1] - r[0]) * 199))
+    if rl_engine != rlist:
+        cnt_o, code_o = O.neighbors(x, lo, hi - lo, pbc, rl_engine, maxn=160)
+    assert np.array_equal(cnt_g, cnt_o)
+    for i in range(len(x)):
+        assert np.array_equal(code_g[i, :cnt_g[i]], code_o[i, :cnt_o[i]]), i
+    # positions come back unchanged (cell-order permutation undone)
+    assert np.array_equal(eng.download(om.X), x)
+    eng.close()
+
+
+# ---------------------------------------------------------------------------
+# EAM (FP32 forces, FP64 integration)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("cells", [6, 16, 40])
+def test_eam_forces_vs_oracle(cells):
+    s = eam_system(cells=cells, seed=12346)
+    say(f"[eam] cells={cells} n={len(s['x'])}")
+    eng = _eam_engine(s)
+    ep, vir = eng.compute()
+    f = eng.download(om.F)
+    pot = eng.download(om.POT)
+    rho = eng.download(om.RHO)
+    fo, poto, viro, rhoo = _oracle_eam(s["x"], s, 5.5 + 0.3)
+    err = rel_rms(f, fo)
+    say(f"[eam] cells={cells} force relRMS={err:.3e} "
+        f"Ep={ep:.10f} oracle={poto.sum():.10f}")
+    assert err <= FORCE_RTOL
+    assert ep == pytest.approx(poto.sum(), rel=1e-6)
+    assert vir == pytest.approx(viro.sum() / 2, rel=1e-5)
+    assert np.abs(pot - poto).max() < 1e-5
+    assert rel_rms(rho, rhoo) < 1e-6
+    eng.close()
+
+
+def test_eam_perfect_crystal():
+    pos, (lo, hi) = lattice.fcc100(8, 8, 8, 4.05)
+    s = dict(x=pos, v=np.zeros_like(pos), lo=lo, hi=hi, L=hi - lo, pbc=7,
+             mass=tables.mass_sim(tables.MASS_AL))
+    eng = _eam_engine(s)
+    ep, _ = eng.compute()
+    f = eng.download(om.F)
+    assert ep / len(pos) == pytest.approx(-3.36, abs=1e-5)
+    assert np.abs(f).max() < 1e-4
+    eng.close()
+
+
+def test_eam_trajectory_vs_oracle():
+    s = eam_system(cells=5, seed=777)
+    say(f"[eam-traj] n={len(s['x'])}")
+    eng = _eam_engine(s, skin=0.4)
+    ep0, _ = eng.compute()
+    eng.zero_forces()                       # FirstRun (system.cpp:872)
+    t = eam_tables()
+    sp = [O.Spline.from_table(t[k]) for k in ("PAIR", "EDENS", "EMBED")]
+    x_o, v_o, en = O.run(1, s["x"], s["v"], s["mass"], s["lo"], s["L"], 7, sp, 5.5, 0.4,
+                         5, 50, 2e-3)
+    ek, ep = eng.step(50, 2e-3)
+    x_g = eng.download(om.X)
+    d = wrap_np(x_g, s["lo"], s["L"], 7) - wrap_np(x_o, s["lo"], s["L"], 7)
+    d -= np.round(d / s["L"]) * s["L"]
+    say(f"[eam-traj] max |dx|={np.abs(d).max():.3e} Ek={ek:.8f}/{en[-1,0]:.8f}")
+    assert np.abs(d).max() < 1e-6
+    assert ek == pytest.approx(en[-1, 0], rel=1e-6)
+    assert ep == pytest.approx(en[-1, 1], rel=1e-7)
+    eng.close()
+
+
+def test_eam_energy_conservation():
+    s = eam_system(cells=8, seed=4242)
+    say(f"[eam-drift] n={len(s['x'])}")
+    eng = _eam_engine(s, skin=0.5)
+    base, _ = eng.compute()
+    eng.zero_forces()
+    e = []
+    for _ in range(20):
+        ek, ep = eng.step(50, 2e-3)
+        e.append((ek + ep - base) / len(s["x"]))
+    e = np.array(e)
+    say(f"[eam-drift] E/N range {np.ptp(e):.3e}")
+    assert np.ptp(e) < 5e-5
+    assert abs(e[-5:].mean() - e[:5].mean()) < 1e-5
+    eng.close()
+
+
+# ---------------------------------------------------------------------------
+# Lennard-Jones, plain method in FP64
+# ---------------------------------------------------------------------------
+def test_lj_fp64_forces_vs_oracle():
+    s = lj_system(cells=10)
+    rng = np.random.default_rng(1)
+    x = wrap_np(s["x"] + rng.normal(0, 0.05, s["x"].shape), s["lo"], s["L"], 7)
+    s["x"] = x
+    eng = _lj_engine(s)
+    ep, vir = eng.compute()
+    f = eng.download(om.F)
+    phi = O.Spline.from_table(tables.lj_table())
+    nb = O.neighbors(x, s["lo"], s["L"], 7, 8.8, maxn=128)
+    fo, poto, viro = O.pair_forces(x, s["lo"], s["L"], 7, nb, phi)
+    err = rel_rms(f, fo)
+    say(f"[lj64] force relRMS={err:.3e}")
+    assert err < 1e-12
+    assert ep == pytest.approx(poto.sum(), rel=1e-13)
+    assert vir == pytest.approx(viro.sum() / 2, rel=1e-12)
+    eng.close()
+
+
+def test_lj_fp32_forces_vs_oracle():
+    s = lj_system(cells=8)
+    rng = np.random.default_rng(2)
+    s["x"] = wrap_np(s["x"] + rng.normal(0, 0.05, s["x"].shape), s["lo"], s["L"], 7)
+    eng = _lj_engine(s, fp64=False)
+    eng.compute()
+    f = eng.download(om.F)
+    phi = O.Spline.from_table(tables.lj_table())
+    nb = O.neighbors(s["x"], s["lo"], s["L"], 7, 8.8, maxn=128)
+    fo, _, _ = O.pair_forces(s["x"], s["lo"], s["L"], 7, nb, phi)
+    assert rel_rms(f, fo) <= FORCE_RTOL
+    eng.close()
+
+
+def test_lj_trajectory_fp64():
+    s = lj_system(cells=10)
+    say(f"[lj64-traj] n={len(s['x'])}")
+    eng = _lj_engine(s)
+    eng.compute()
+    eng.zero_forces()
+    phi = O.Spline.from_table(tables.lj_table())
+    x_o, v_o, en = O.run(0, s["x"], s["v"], s["mass"], s["lo"], s["L"], 7, [phi], 8.5, 0.3,
+                         10, 200, 1e-3)
+    ek, ep = eng.step(200, 1e-3)
+    x_g = eng.download(om.X)
+    v_g = eng.download(om.V)
+    d = x_g - x_o
+    d -= np.round(d / s["L"]) * s["L"]
+    say(f"[lj64-traj] max |dx|={np.abs(d).max():.3e} |dv|={np.abs(v_g - v_o).max():.3e}")
+    assert np.abs(d).max() < 1e-9
+    assert ek == pytest.approx(en[-1, 0], rel=1e-10)
+    assert ep == pytest.approx(en[-1, 1], rel=1e-10)
+    eng.close()
+
+
+# ---------------------------------------------------------------------------
+# error paths (forced from the host; every kernel exits normally)
+# ---------------------------------------------------------------------------
+def test_overlap_detected():
+    s = lj_system(cells=4)
+    x = s["x"].copy()
+    x[1] = x[0] + 1e-4
+    s["x"] = x
+    eng = _lj_engine(s)
+    with pytest.raises(om.OmdgError) as e:
+        eng.compute()
+    assert e.value.code == om.EOVERLAP
+    eng.close()
+
+
+def test_table_low_range_detected():
+    s = lj_system(cells=4)
+    x = s["x"].copy()
+    x[1] = x[0] + np.array([1.0, 0.0, 0.0])   # below the table start of 2 A
+    s["x"] = x
+    eng = _lj_engine(s)
+    with pytest.raises(om.OmdgError) as e:
+        eng.compute()
+    assert e.value.code == om.ERANGE_LO
+    eng.close()
+
+
+def test_state_roundtrip_after_steps():
+    s = eam_system(cells=4)
+    eng = _eam_engine(s)
+    eng.compute()
+    eng.step(30, 2e-3)
+    x = eng.download(om.X)
+    v = eng.download(om.V)
+    eng.set_state(x, v)
+    assert np.array_equal(eng.download(om.X), x)
+    assert np.array_equal(eng.download(om.V), v)
+    eng.close()

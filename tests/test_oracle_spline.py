@@ -1,0 +1,115 @@
+"""Pins the oracle's spline (TableReader restatement, treader.cpp:37-264)
+against what the mathematics fixes and against the reference's table files."""
+import os
+
+import numpy as np
+import pytest
+
+from tests.common import GOLDEN
+
+
+def test_quadratic_reproduced_exactly(orc):
+    # parabolic run-out: S_0 = S_1 and S_n = S_{n-1} hold for quadratic data,
+    # so the spline IS the quadratic
+    r0, dr, n = 0.7, 0.013, 400
+    r = r0 + dr * np.arange(n)
+    y = 2.0 - 1.5 * r + 0.75 * r * r
+    sp = orc.Spline(y, dr, r0)
+    for rr in np.linspace(r0, r0 + dr * (n - 1), 997):
+        rc, v, dv = sp.read(rr)
+        assert rc == 0
+        assert v == pytest.approx(2.0 - 1.5 * rr + 0.75 * rr * rr, rel=1e-11, abs=1e-11)
+        assert dv == pytest.approx(-1.5 + 1.5 * rr, rel=1e-9, abs=1e-9)
+
+
+def test_cubic_is_not_reproduced(orc):
+    # a plausible mistake (natural end conditions M=0) would still reproduce
+    # linear data; cubic data must show the run-out end error
+    r0, dr, n = 0.0, 0.1, 50
+    r = r0 + dr * np.arange(n)
+    sp = orc.Spline(r ** 3, dr, r0)
+    _, v, _ = sp.read(0.05)
+    assert abs(v - 0.05 ** 3) > 1e-6
+
+
+def test_interpolates_knots_and_continuity(orc):
+    rng = np.random.default_rng(3)
+    n, dr, r0 = 60, 0.05, 1.0
+    y = np.cumsum(rng.normal(size=n))
+    sp = orc.Spline(y, dr, r0)
+    c = sp.coef
+    for i in range(n - 1):
+        assert c[i, 3] == y[i]
+        # value and slopes continuous at knot i+1
+        dx = dr
+        v_left = c[i, 3] + dx * (c[i, 2] + dx * (c[i, 1] + dx * c[i, 0]))
+        d_left = c[i, 2] + dx * (2 * c[i, 1] + 3 * dx * c[i, 0])
+        dd_left = 2 * c[i, 1] + 6 * dx * c[i, 0]
+        assert v_left == pytest.approx(y[i + 1], abs=1e-9)
+        if i + 1 < n - 1:
+            assert d_left == pytest.approx(c[i + 1, 2], abs=1e-8)
+            assert dd_left == pytest.approx(2 * c[i + 1, 1], abs=1e-6)
+    # end conditions: second derivative equal on the first two and last two knots
+    assert c[0, 1] == pytest.approx(c[1, 1], abs=1e-10)
+    s_last = c[n - 2, 1] + 3 * dr * c[n - 2, 0]
+    assert s_last == pytest.approx(c[n - 2, 1], abs=1e-8)
+
+
+def test_edge_branches_and_errors(orc):
+    n, dr, r0 = 20, 0.1, 1.0
+    y = np.linspace(0.0, 1.0, n) ** 2
+    sp = orc.Spline(y, dr, r0)
+    top = r0 + dr * (n - 1)
+    rc, v, _ = sp.read(top)                 # x == n-1: last segment, dx = dr
+    assert rc == 0 and v == pytest.approx(y[-1], abs=1e-12)
+    rc, v, _ = sp.read(top + 0.5 * dr)      # still index n-1
+    assert rc == 0
+    assert sp.read(top + 1.01 * dr)[0] == 1  # index n: high limit exceeded
+    rc, v, _ = sp.read(r0 - 0.5 * dr)        # (int) truncates to 0: segment 0
+    c = sp.coef
+    dx = -0.5 * dr
+    assert v == pytest.approx(c[0, 3] + dx * (c[0, 2] + dx * (c[0, 1] + dx * c[0, 0])))
+    rc, v, _ = sp.read(r0 - 1.5 * dr)        # index -1: segment 0 at dx = rval + dr
+    dx = -0.5 * dr
+    assert rc == 0
+    assert v == pytest.approx(c[0, 3] + dx * (c[0, 2] + dx * (c[0, 1] + dx * c[0, 0])))
+    assert sp.read(r0 - 2.5 * dr)[0] == 2    # below: low limit exceeded
+
+
+def test_rmult(orc):
+    n, dr, r0 = 100, 0.05, 1.0
+    r = r0 + dr * np.arange(n)
+    s = np.sin(r)
+    plain = orc.Spline(s, dr, r0)
+    rm = orc.Spline(s, dr, r0, rmult=True)
+    for rr in (1.03, 2.5, 5.9):
+        _, sv, sd = plain.read(rr)
+        rc, v, dv = rm.read(rr)
+        assert v == pytest.approx(sv / rr, rel=1e-15)
+        assert dv == pytest.approx((sd - sv / rr) / rr, rel=1e-14)
+        assert rm.value(rr)[1] == pytest.approx(sv / rr, rel=1e-15)
+
+
+def test_reference_table_files(orc, tmp_path):
+    """The parser reads the reference's own table format: named blocks
+    (eam.*), an unnamed block (pair.*), PLAIN and RMULT."""
+    zbl = os.path.join(GOLDEN, "ref", "pair.ar-si.zbl")
+    y, dr, r0, rmult = orc.table_load(zbl, "")
+    assert len(y) == 1000 and dr == 0.0051 and r0 == 0.0051 and not rmult
+    assert y[0] == 678380 and y[3] == 148267
+    from objectmd_amd import tables
+    t = tables.JohnsonEAM().tables(200)
+    p = tmp_path / "eam.test"
+    tables.write_table_file(str(p), t)
+    for name in ("PAIR", "EDENS", "EMBED"):
+        y, dr, r0, rm = orc.table_load(str(p), name)
+        assert np.array_equal(y, t[name].y)
+        assert dr == t[name].dr and r0 == t[name].r0
+
+
+@pytest.mark.skipif(not os.path.exists("/root/reference/tables/eam.aluminum"),
+                    reason="reference table files not present")
+def test_reference_eam_aluminum_header(orc):
+    y, dr, r0, rm = orc.table_load("/root/reference/tables/eam.aluminum", "PAIR")
+    assert len(y) == 1000 and rm and dr == 5.558e-03 and r0 == 5.558e-03
+    assert y[0] == 2.315823166114421e+03
