@@ -408,6 +408,8 @@ int omdg_destroy(omdg_t h) {
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   free_atoms(c);
   dfree(c.nbr);
+  dfree(c.bond_b);
+  dfree(c.bond_p);
   dfree(c.cell_start);
   dfree(c.cell_fill);
   dfree(c.tab32);

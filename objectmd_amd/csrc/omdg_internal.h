@@ -107,6 +107,9 @@ struct Ctx {
   float* rho = nullptr;
   float* fp = nullptr;  // F'(rho)
   float* emb = nullptr; // F(rho)
+  float* bond_b = nullptr;  // Tersoff per-slot bond order (ELL like nbr)
+  float* bond_p = nullptr;  // Tersoff per-slot zeta prefactor
+  size_t cap_bond = 0;
   int cap_atoms = 0;
   // neighbour list (ELL, column-major: nbr[k * n + i])
   uint32_t* nbr = nullptr;

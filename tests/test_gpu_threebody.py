@@ -1,0 +1,147 @@
+"""GPU parity of the three-body kernels (Stillinger-Weber with its ZBL core
+and mixed pair tables, Tersoff) against the FP64 oracle."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import objectmd_amd as om
+from objectmd_amd import lattice, tables
+from oracle import oracle as O
+from tests.common import GOLDEN, rel_rms, sw_params, tersoff_params, wrap_np
+
+pytestmark = pytest.mark.gpu
+FORCE_RTOL = 1e-5
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _si(cells, disp, seed, pbc, a=5.431):
+    pos, (lo, hi) = lattice.diamond100(cells, cells, cells, a)
+    rng = np.random.default_rng(seed)
+    x = wrap_np(pos + rng.normal(0, disp, pos.shape), lo, hi - lo, pbc)
+    return x, lo, hi
+
+
+def _sw_param_oracle():
+    p = sw_params()
+    return O.SwParam(*p[:10], int(p[10]), *p[11:])
+
+
+@pytest.mark.parametrize("pbc", [7, 3])
+def test_sw_forces_vs_oracle(pbc):
+    x, lo, hi = _si(6, 0.1, 31 + pbc, pbc)
+    say(f"[sw] n={len(x)} pbc={pbc}")
+    eng = om.Engine()
+    eng.set_box(lo, hi, pbc)
+    eng.upload_atoms(x, mass=[tables.mass_sim(tables.MASS_SI)])
+    eng.set_sw(sw_params())
+    eng.set_potential(om.POT_SW)
+    eng.set_neighbor(0.3, 10, 0)
+    ep, _ = eng.compute()
+    f = eng.download(om.F)
+    nb = O.neighbors(x, lo, hi - lo, pbc, 3.771 + 0.3, maxn=64)
+    fo, po = O.sw_forces(x, lo, hi - lo, pbc, nb, _sw_param_oracle())
+    err = rel_rms(f, fo)
+    say(f"[sw] relRMS={err:.3e} Ep={ep:.8f} oracle={po.sum():.8f}")
+    assert err <= FORCE_RTOL
+    assert ep == pytest.approx(po.sum(), rel=1e-6)
+    assert np.abs(eng.download(om.POT) - po).max() < 1e-5
+    eng.close()
+
+
+def test_sw_zbl_core_and_ar_si_table():
+    """Compressed Si pairs (ZBL core and splice) plus one Ar atom interacting
+    through the reference's pair.ar-si.zbl table."""
+    x, lo, hi = _si(5, 0.05, 5, 7)
+    x = x.copy()
+    x[1] = x[0] + np.array([1.1, 0.3, 0.2])     # inside sx0: ZBL core
+    x[9] = x[8] + np.array([1.5, 0.4, 0.0])     # between sx0 and sx1: splice
+    tid = np.zeros(len(x), np.int32)
+    ar = len(x) // 2
+    tid[ar] = 1
+    x[ar] = x[ar] + np.array([0.4, 0.1, 0.0])
+    say(f"[sw-zbl] n={len(x)}")
+    y, dr, r0, rm = O.table_load(os.path.join(GOLDEN, "ref", "pair.ar-si.zbl"), "")
+    eng = om.Engine()
+    eng.set_box(lo, hi, 7)
+    eng.upload_atoms(x, tid=tid, mass=[tables.mass_sim(tables.MASS_SI),
+                                       tables.mass_sim(tables.MASS_AR)])
+    eng.add_table(om.TAB_PAIR, 0, 1, y, dr, r0, rm)
+    eng.set_sw(sw_params())
+    eng.set_potential(om.POT_SW)
+    eng.set_neighbor(0.3, 10, 0)
+    eng.compute()
+    f = eng.download(om.F)
+    # oracle: SW among Si atoms + table for the Ar-Si pairs
+    rl = max(3.771, r0 + dr * (len(y) - 1)) + 0.3
+    cnt, code = O.neighbors(x, lo, hi - lo, 7, rl, maxn=96)
+    cs = np.zeros_like(cnt)
+    codes = np.zeros_like(code)
+    ca = np.zeros_like(cnt)
+    codea = np.zeros_like(code)
+    for i in range(len(x)):
+        for c in code[i, :cnt[i]]:
+            j = c // 27
+            if tid[i] == 0 and tid[j] == 0:
+                codes[i, cs[i]] = c
+                cs[i] += 1
+            elif tid[i] != tid[j]:
+                codea[i, ca[i]] = c
+                ca[i] += 1
+    fo_sw, _ = O.sw_forces(x, lo, hi - lo, 7, (cs, codes), _sw_param_oracle())
+    fo_t, _, _ = O.pair_forces(x, lo, hi - lo, 7, (ca, codea), O.Spline(y, dr, r0, rm))
+    fo = fo_sw + fo_t
+    err = rel_rms(f, fo)
+    say(f"[sw-zbl] relRMS={err:.3e}")
+    assert err <= FORCE_RTOL
+    eng.close()
+
+
+def _ters_oracle(R, D):
+    return O.TersoffParam(*tersoff_params(R, D), 3)
+
+
+@pytest.mark.parametrize("pbc,R,D", [(3, 2.7, 0.15), (7, 3.0, 0.2)])
+def test_tersoff_forces_vs_oracle(pbc, R, D):
+    x, lo, hi = _si(6, 0.12, 77 + pbc, pbc, a=5.432)
+    say(f"[tersoff] n={len(x)} pbc={pbc} R={R} D={D}")
+    eng = om.Engine()
+    eng.set_box(lo, hi, pbc)
+    eng.upload_atoms(x, mass=[tables.mass_sim(tables.MASS_SI)])
+    eng.set_tersoff(tersoff_params(R, D))
+    eng.set_potential(om.POT_TERSOFF)
+    eng.set_neighbor(0.3, 10, 0)
+    ep, _ = eng.compute()
+    f = eng.download(om.F)
+    nb = O.neighbors(x, lo, hi - lo, pbc, R + D + 0.3, maxn=64)
+    fo, po = O.tersoff_forces(x, lo, hi - lo, pbc, nb, _ters_oracle(R, D))
+    err = rel_rms(f, fo)
+    say(f"[tersoff] relRMS={err:.3e} Ep={ep:.8f} oracle={po.sum():.8f}")
+    assert err <= FORCE_RTOL
+    assert ep == pytest.approx(po.sum(), rel=1e-6)
+    eng.close()
+
+
+def test_tersoff_steps_conserve_energy():
+    x, lo, hi = _si(5, 0.05, 3, 3, a=5.432)
+    m = tables.mass_sim(tables.MASS_SI)
+    v = lattice.maxwell_boltzmann(len(x), m, 1000.0, 4)
+    eng = om.Engine()
+    eng.set_box(lo, hi, 3)
+    eng.upload_atoms(x, v, mass=[m])
+    eng.set_tersoff(tersoff_params(2.7, 0.15))
+    eng.set_potential(om.POT_TERSOFF)
+    eng.set_neighbor(0.4, 5, 0)
+    base, _ = eng.compute()
+    eng.zero_forces()
+    e = []
+    for _ in range(10):
+        ek, ep = eng.step(50, 0.001)
+        e.append((ek + ep - base) / len(x))
+    say(f"[tersoff-drift] E/N range {np.ptp(e):.3e}")
+    assert np.ptp(e) < 2e-4
+    eng.close()
