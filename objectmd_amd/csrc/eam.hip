@@ -88,8 +88,7 @@ __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
   const float4 qi = a.q[s];
   const float fpi = a.fp[s];
   const int nn = a.nnbr[s];
-  Kahan fx, fy, fz;
-  float pot = 0.f, vir = 0.f;
+  Kahan fx, fy, fz, pot, vir;
   for (int k = 0; k < nn; k++) {
     const uint32_t e = a.nbr[(size_t)k * n + s];
     const uint32_t t = e & kSlotMask;
@@ -109,8 +108,8 @@ __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
       fy.add(fr * dy);
       fz.add(fr * dz);
       if (ENERGY) {
-        pot += 0.5f * p;
-        vir += fr * r2;
+        pot.add(0.5f * p);
+        vir.add(fr * r2);
       }
     }
   }
@@ -118,8 +117,8 @@ __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
   a.f[(size_t)n + s] = fy.s;
   a.f[2 * (size_t)n + s] = fz.s;
   if (ENERGY) {
-    a.pot[s] = pot + a.emb[s];
-    a.vir[s] = vir;
+    a.pot[s] = pot.s + a.emb[s];
+    a.vir[s] = vir.s;
   }
 }
 

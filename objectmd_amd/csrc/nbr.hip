@@ -300,10 +300,12 @@ void nbr_build(Ctx& c) {
   // 1. wrap + bounding box
   k_wrap_aabb<<<nb, B, 0, c.stream>>>(c.x[c.cur], n, make_double3(c.lo[0], c.lo[1], c.lo[2]),
                                       make_double3(c.L[0], c.L[1], c.L[2]), c.pbc, part, c.dstat);
-  k_aabb_final<<<1, 64, 0, c.stream>>>(part, nb, c.aabb);
-  double bb[6];
-  OMDG_HIP(hipMemcpyAsync(bb, c.aabb, sizeof bb, hipMemcpyDeviceToHost, c.stream));
-  OMDG_HIP(hipStreamSynchronize(c.stream));
+  double bb[6] = {0, 0, 0, 0, 0, 0};
+  if (c.pbc != 7) {  // free dimensions need the bounding box on the host
+    k_aabb_final<<<1, 64, 0, c.stream>>>(part, nb, c.aabb);
+    OMDG_HIP(hipMemcpyAsync(bb, c.aabb, sizeof bb, hipMemcpyDeviceToHost, c.stream));
+    OMDG_HIP(hipStreamSynchronize(c.stream));
+  }
   // 2. grid: cells no smaller than rlist (VerletList.cpp:115-117); periodic
   //    dimensions tile the box exactly so the stencil wraps consistently
   Grid& g = c.grid;

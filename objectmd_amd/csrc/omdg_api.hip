@@ -809,18 +809,28 @@ int omdg_step(omdg_t h, int nsteps, double dt, double* ekin, double* epot) {
       // the first half kick needs forces of the current positions
       compute_forces(c, false);
     }
+    // The list stays valid while no atom has moved skin/2 since the build.
+    // At a check the host reads the largest displacement and speed; it
+    // rebuilds if one more step could cross skin/2, and otherwise schedules
+    // the next check before the fastest atom, at 1.5x its speed, could.
+    int until_check = 1;
     for (int s = 0; s < nsteps; s++) {
       {
         TimerScope ts(c, TG_DRIFT);
         integrate_kick_drift(c, dt);
       }
-      if ((s + 1) % c.check_every == 0) {
+      if (--until_check == 0) {
         read_status(c);
         raise_device_errors(c);
-        const double disp = std::sqrt((double)bits_to_float(c.hstat[ST_DISP2]));
+        double disp = std::sqrt((double)bits_to_float(c.hstat[ST_DISP2]));
         const double vmax = std::sqrt((double)bits_to_float(c.hstat[ST_VMAX2]));
-        // rebuild before any atom can cover skin/2 before the next check
-        if (disp + 1.5 * c.check_every * vmax * dt >= 0.5 * c.skin) rebuild(c);
+        const double per_step = std::max(1.5 * vmax * dt, 1e-12);
+        if (disp + per_step >= 0.5 * c.skin) {
+          rebuild(c);
+          disp = 0.0;
+        }
+        const double k = std::floor((0.5 * c.skin - disp) / per_step);
+        until_check = (int)std::max(1.0, std::min((double)c.check_every, k));
       }
       const bool last = (s == nsteps - 1);
       compute_forces(c, last);

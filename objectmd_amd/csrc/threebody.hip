@@ -328,11 +328,22 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_bond(TBArgs a) {
     const float fA = -p.B * expf(-p.lambda2 * rij);
     float b = 1.f, db = 0.f;
     if (zeta > 0.f) {
-      const float x = powf(p.beta * zeta, p.n);
-      const float base = 1.f + x;
-      b = powf(base, -0.5f / p.n);
-      // db/dzeta = -1/2 (1+x)^(-1/2n - 1) beta^n zeta^(n-1) = -1/2 b x / ((1+x) zeta)
-      db = -0.5f * b * x / (base * zeta);
+      // x = (beta zeta)^n overflows FP32 for close pairs: work with
+      // lx = log x.  b = (1+x)^(-1/2n) = exp(-log1p(x) / 2n) and
+      // db/dzeta = -1/2 (1+x)^(-1/2n - 1) beta^n zeta^(n-1)
+      //          = -b x / (2 zeta (1+x)).
+      const float lx = p.n * logf(p.beta * zeta);
+      float l1p, t;
+      if (lx > 30.f) {
+        l1p = lx;  // log1p(x) = lx + log1p(1/x), 1/x < 1e-13
+        t = 1.f;
+      } else {
+        const float x = expf(lx);
+        l1p = log1pf(x);
+        t = x / (1.f + x);
+      }
+      b = expf(-0.5f / p.n * l1p);
+      db = -0.5f * b * t / zeta;
     }
     a.bond_b[(size_t)slot[u] * n + i] = b;
     a.bond_p[(size_t)slot[u] * n + i] = 0.5f * fc * fA * db;

@@ -129,7 +129,10 @@ def test_eam_forces_vs_oracle(cells):
         f"Ep={ep:.10f} oracle={poto.sum():.10f}")
     assert err <= FORCE_RTOL
     assert ep == pytest.approx(poto.sum(), rel=1e-6)
-    assert vir == pytest.approx(viro.sum() / 2, rel=1e-5)
+    # the pair virials cancel to a small total (near-zero pressure) and each
+    # FP32 pair term carries the cancellation of the embedding and pair
+    # parts of fr: bound the error by the size of the per-atom terms
+    assert abs(vir - viro.sum() / 2) <= 2e-5 * np.abs(viro).sum()
     assert np.abs(pot - poto).max() < 1e-5
     assert rel_rms(rho, rhoo) < 1e-6
     eng.close()
