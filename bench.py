@@ -56,6 +56,26 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def copy_probe(mbytes):
+    """One device-to-device hipMemcpy of mbytes MiB: the plain copy kernel
+    whose FETCH_SIZE / WRITE_SIZE counters calibrate the traffic figures."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = int(mbytes) << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) != 0:   # DeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"[bench] copy probe: {nbytes} bytes device to device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +87,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20,
                     help="steps of the FP64 oracle timed on the host (0: skip)")
     ap.add_argument("--no-force-check", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB "
+                         "(counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus above 1 is not supported "
@@ -107,6 +130,8 @@ def main():
     rate = n * args.steps / elapsed
     log(f"[bench] {args.steps} steps in {elapsed:.4f} s: {rate:.4e} atom-steps/s")
     e_end = (ek + ep - base) / n
+    if args.copy_probe_mb:
+        copy_probe(args.copy_probe_mb)
 
     force_ms, force_launches = times["force"]
     avg_force_s = force_ms / max(force_launches, 1) / 1e3

@@ -14,8 +14,20 @@
 // Pair displacements come from FP32 cell-relative snapshots:
 // d = q_i - q_j - o * h with o the stencil offset of the list entry, so the
 // rounding is that of |q| ~ one cell, not of the absolute coordinate.
-// Per-atom sums are Kahan-compensated FP32.
+//
+// LANES lanes serve one atom: lane j walks list entries j, j+LANES, ... with
+// a Kahan-compensated FP32 sum, and the compensated partials are added in
+// FP64 across the lanes.  Each lane requests the list words, snapshots and
+// table records of CHUNK entries before using any of them, so a wavefront
+// keeps many independent loads in flight; the segment lookup is branch-free
+// and range errors reach the status word once per lane.  When PAIR and
+// EDENS share one grid their coefficients are interleaved per interval
+// (32 B, one cache line per pair lookup).  (LANES, CHUNK) is picked per
+// context by timing the candidates on the first evaluation.
 #include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
 
 #include "omdg_internal.h"
 #include "spline_dev.h"
@@ -23,6 +35,10 @@
 namespace omdg {
 
 namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int kCentre = 13;  // image code of the zero offset
 
 struct EamArgs {
   const float4* q;
@@ -39,87 +55,268 @@ struct EamArgs {
   float cut_e2, cut_p2;
   float hx, hy, hz;
   TabView phi, edens, embed;
+  const float4* pack;  // {phi a,b,c,d}, {rho a,b,c,d} per interval, or null
 };
 
-constexpr int kBlock = 256;
+// Compensated partial -> FP64, summed over the LANES lanes of one atom.
+template <int LANES>
+__device__ inline double lanes_sum(const Kahan& k) {
+  double v = (double)k.s - (double)k.c;
+#pragma unroll
+  for (int o = 1; o < LANES; o <<= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
 
-template <bool ENERGY>
+template <bool ENERGY, int LANES, int CHUNK>
 __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
   __shared__ float4 offs[27];
   load_offsets(offs, a.hx, a.hy, a.hz);
   __syncthreads();
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.n) return;
+  const int g = xcd_block() * blockDim.x + threadIdx.x;
+  const int s = g / LANES, j = g % LANES;
   const int n = a.n;
-  const float4 qi = a.q[s];
-  const int nn = a.nnbr[s];
+  const bool live = s < n;  // no early exit: all lanes take part in the shuffles
+  const float4 qi = live ? a.q[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const int nn = live ? a.nnbr[s] : 0;
+  int err = 0;
   Kahan rho;
-  for (int k = 0; k < nn; k++) {
-    const uint32_t e = a.nbr[(size_t)k * n + s];
-    const float4 qj = a.q[e & kSlotMask];
-    const float4 o = offs[e >> kImgShift];
-    const float dx = qi.x - qj.x - o.x;
-    const float dy = qi.y - qj.y - o.y;
-    const float dz = qi.z - qj.z - o.z;
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 <= a.cut_e2) {
-      float v;
-      spline_value32(a.edens, sqrtf(r2), v, a.err);
-      rho.add(v);
+  for (int k0 = j; k0 < nn; k0 += LANES * CHUNK) {
+    uint32_t e[CHUNK];
+#pragma unroll
+    for (int u = 0; u < CHUNK; u++) {
+      const int k = k0 + u * LANES;
+      e[u] = (k < nn) ? a.nbr[nidx(s, k, n)] : kNone;
+    }
+    float4 qj[CHUNK];
+#pragma unroll
+    for (int u = 0; u < CHUNK; u++) qj[u] = (e[u] != kNone) ? a.q[e[u] & kSlotMask] : qi;
+    float r[CHUNK], dx[CHUNK];
+    bool in[CHUNK];
+    int seg[CHUNK];
+#pragma unroll
+    for (int u = 0; u < CHUNK; u++) {
+      const float4 o = offs[e[u] != kNone ? (e[u] >> kImgShift) : kCentre];
+      const float ex = qi.x - qj[u].x - o.x;
+      const float ey = qi.y - qj[u].y - o.y;
+      const float ez = qi.z - qj[u].z - o.z;
+      const float r2 = ex * ex + ey * ey + ez * ez;
+      in[u] = (e[u] != kNone) && (r2 <= a.cut_e2);
+      r[u] = in[u] ? r2 * rsqrtf(r2) : a.edens.r0;
+      seg[u] = spline_locate32_nb(a.edens, r[u], dx[u], err);
+    }
+    float4 c[CHUNK];
+#pragma unroll
+    for (int u = 0; u < CHUNK; u++) c[u] = a.edens.c32[seg[u]];
+#pragma unroll
+    for (int u = 0; u < CHUNK; u++) {
+      float v = c[u].w + dx[u] * (c[u].z + dx[u] * (c[u].y + dx[u] * c[u].x));
+      if (a.edens.rmult) v /= r[u];
+      rho.add(in[u] ? v : 0.f);
     }
   }
-  const float r = rho.s;
-  a.rho[s] = r;
-  // embedding function and derivative in FP64 (once per atom)
-  double F, dF;
-  spline_read64(a.embed, (double)r, F, dF, a.err);
-  a.fp[s] = (float)dF;
-  if (ENERGY) a.emb[s] = (float)F;
+  if (err) atomicOr(a.err, err);
+  const double rs = lanes_sum<LANES>(rho);
+  if (live && j == 0) {
+    a.rho[s] = (float)rs;
+    // embedding function and derivative in FP64, once per atom
+    double F, dF;
+    spline_read64(a.embed, (double)(float)rs, F, dF, a.err);
+    a.fp[s] = (float)dF;
+    if (ENERGY) a.emb[s] = (float)F;
+  }
 }
 
-template <bool ENERGY>
+template <bool ENERGY, bool PACKED, int LANES, int CHUNK>
 __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
   __shared__ float4 offs[27];
   load_offsets(offs, a.hx, a.hy, a.hz);
   __syncthreads();
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.n) return;
+  const int g = xcd_block() * blockDim.x + threadIdx.x;
+  const int s = g / LANES, j = g % LANES;
   const int n = a.n;
-  const float4 qi = a.q[s];
-  const float fpi = a.fp[s];
-  const int nn = a.nnbr[s];
+  const bool live = s < n;
+  const float4 qi = live ? a.q[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float fpi = live ? a.fp[s] : 0.f;
+  const int nn = live ? a.nnbr[s] : 0;
+  int err = 0;
   Kahan fx, fy, fz, pot, vir;
-  for (int k = 0; k < nn; k++) {
-    const uint32_t e = a.nbr[(size_t)k * n + s];
-    const uint32_t t = e & kSlotMask;
-    const float4 qj = a.q[t];
-    const float4 o = offs[e >> kImgShift];
-    const float dx = qi.x - qj.x - o.x;
-    const float dy = qi.y - qj.y - o.y;
-    const float dz = qi.z - qj.z - o.z;
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 <= a.cut_p2) {
-      const float r = sqrtf(r2);
-      float p, dp, rv, drho = 0.f;
-      spline_read32(a.phi, r, p, dp, a.err);
-      if (r2 <= a.cut_e2) spline_read32(a.edens, r, rv, drho, a.err);
-      const float fr = -((fpi + a.fp[t]) * drho + dp) / r;
-      fx.add(fr * dx);
-      fy.add(fr * dy);
-      fz.add(fr * dz);
+  for (int k0 = j; k0 < nn; k0 += LANES * CHUNK) {
+    uint32_t e[CHUNK];
+#pragma unroll
+    for (int u = 0; u < CHUNK; u++) {
+      const int k = k0 + u * LANES;
+      e[u] = (k < nn) ? a.nbr[nidx(s, k, n)] : kNone;
+    }
+    float4 qj[CHUNK];
+    float fpj[CHUNK];
+#pragma unroll
+    for (int u = 0; u < CHUNK; u++) {
+      const bool v = e[u] != kNone;
+      qj[u] = v ? a.q[e[u] & kSlotMask] : qi;
+      fpj[u] = v ? a.fp[e[u] & kSlotMask] : 0.f;
+    }
+    float d[CHUNK][3], r2[CHUNK], rinv[CHUNK], dxp[CHUNK], dxe[CHUNK];
+    bool in[CHUNK], ine[CHUNK];
+    int sp[CHUNK], se[CHUNK];
+#pragma unroll
+    for (int u = 0; u < CHUNK; u++) {
+      const float4 o = offs[e[u] != kNone ? (e[u] >> kImgShift) : kCentre];
+      d[u][0] = qi.x - qj[u].x - o.x;
+      d[u][1] = qi.y - qj[u].y - o.y;
+      d[u][2] = qi.z - qj[u].z - o.z;
+      r2[u] = d[u][0] * d[u][0] + d[u][1] * d[u][1] + d[u][2] * d[u][2];
+      in[u] = (e[u] != kNone) && (r2[u] <= a.cut_p2);
+      ine[u] = in[u] && (r2[u] <= a.cut_e2);
+      rinv[u] = in[u] ? rsqrtf(r2[u]) : 1.f;
+      const float r = in[u] ? r2[u] * rinv[u] : a.phi.r0;
+      sp[u] = spline_locate32_nb(a.phi, r, dxp[u], err);
+      if (PACKED) {
+        se[u] = sp[u];
+        dxe[u] = dxp[u];
+      } else {
+        se[u] = spline_locate32_nb(a.edens, ine[u] ? r : a.edens.r0, dxe[u], err);
+      }
+    }
+    float4 cp[CHUNK], ce[CHUNK];
+#pragma unroll
+    for (int u = 0; u < CHUNK; u++) {
+      if (PACKED) {
+        cp[u] = a.pack[2 * sp[u]];
+        ce[u] = a.pack[2 * sp[u] + 1];
+      } else {
+        cp[u] = a.phi.c32[sp[u]];
+        ce[u] = a.edens.c32[se[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < CHUNK; u++) {
+      const float x = dxp[u];
+      float p = cp[u].w + x * (cp[u].z + x * (cp[u].y + x * cp[u].x));
+      float dp = cp[u].z + x * (2.0f * cp[u].y + 3.0f * x * cp[u].x);
+      if (a.phi.rmult) {
+        p *= rinv[u];
+        dp = (dp - p) * rinv[u];
+      }
+      const float y = dxe[u];
+      float drho = ce[u].z + y * (2.0f * ce[u].y + 3.0f * y * ce[u].x);
+      if (a.edens.rmult) {
+        const float v = (ce[u].w + y * (ce[u].z + y * (ce[u].y + y * ce[u].x))) * rinv[u];
+        drho = (drho - v) * rinv[u];
+      }
+      drho = ine[u] ? drho : 0.f;  // GetRhoDeriv beyond the EDENS range (team.cpp:130)
+      const float fr = in[u] ? -((fpi + fpj[u]) * drho + dp) * rinv[u] : 0.f;
+      fx.add(fr * d[u][0]);
+      fy.add(fr * d[u][1]);
+      fz.add(fr * d[u][2]);
       if (ENERGY) {
-        pot.add(0.5f * p);
-        vir.add(fr * r2);
+        pot.add(in[u] ? 0.5f * p : 0.f);
+        vir.add(fr * r2[u]);
       }
     }
   }
-  a.f[s] = fx.s;
-  a.f[(size_t)n + s] = fy.s;
-  a.f[2 * (size_t)n + s] = fz.s;
+  if (err) atomicOr(a.err, err);
+  const double sx = lanes_sum<LANES>(fx), sy = lanes_sum<LANES>(fy), sz = lanes_sum<LANES>(fz);
+  double sp = 0.0, sv = 0.0;
   if (ENERGY) {
-    a.pot[s] = pot.s + a.emb[s];
-    a.vir[s] = vir.s;
+    sp = lanes_sum<LANES>(pot);
+    sv = lanes_sum<LANES>(vir);
   }
+  if (live && j == 0) {
+    a.f[s] = (float)sx;
+    a.f[(size_t)n + s] = (float)sy;
+    a.f[2 * (size_t)n + s] = (float)sz;
+    if (ENERGY) {
+      a.pot[s] = (float)(sp + (double)a.emb[s]);
+      a.vir[s] = (float)sv;
+    }
+  }
+}
+
+// Interleave the PAIR and EDENS coefficients when they share one grid.
+void build_pack(Ctx& c, int tp, int te) {
+  c.eam_pack_ok = true;
+  const DevTable& P = c.tables[tp];
+  const DevTable& E = c.tables[te];
+  if (c.eam_pack) (void)hipFree(c.eam_pack);
+  c.eam_pack = nullptr;
+  if (P.n != E.n || P.dr != E.dr || P.r0 != E.r0) return;
+  std::vector<float4> pk(2 * (size_t)P.n);
+  for (int i = 0; i < P.n; i++) {
+    const double* p = &c.host_c64[4 * ((size_t)P.off + i)];
+    const double* e = &c.host_c64[4 * ((size_t)E.off + i)];
+    pk[2 * i] = make_float4((float)p[0], (float)p[1], (float)p[2], (float)p[3]);
+    pk[2 * i + 1] = make_float4((float)e[0], (float)e[1], (float)e[2], (float)e[3]);
+  }
+  OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.eam_pack), sizeof(float4) * pk.size()));
+  OMDG_HIP(hipMemcpy(c.eam_pack, pk.data(), sizeof(float4) * pk.size(), hipMemcpyHostToDevice));
+}
+
+// Candidate (LANES, CHUNK) shapes of the two passes.
+struct Shape {
+  int lanes, chunk;
+};
+constexpr Shape kShapes[] = {{4, 2}, {4, 4}, {8, 1}, {8, 2}, {2, 4}};
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+template <int L, int C>
+void launch_shape(Ctx& c, const EamArgs& a, bool energy, bool pass1, bool pass2) {
+  const int nb = grid_blocks((long)c.n * L, kBlock);
+  if (pass1) {
+    TimerScope ts(c, TG_PASS1);
+    if (energy) k_eam_density<true, L, C><<<nb, kBlock, 0, c.stream>>>(a);
+    else k_eam_density<false, L, C><<<nb, kBlock, 0, c.stream>>>(a);
+  }
+  if (pass2) {
+    TimerScope ts(c, TG_FORCE);
+    if (a.pack) {
+      if (energy) k_eam_force<true, true, L, C><<<nb, kBlock, 0, c.stream>>>(a);
+      else k_eam_force<false, true, L, C><<<nb, kBlock, 0, c.stream>>>(a);
+    } else {
+      if (energy) k_eam_force<true, false, L, C><<<nb, kBlock, 0, c.stream>>>(a);
+      else k_eam_force<false, false, L, C><<<nb, kBlock, 0, c.stream>>>(a);
+    }
+  }
+}
+
+void launch(Ctx& c, const EamArgs& a, bool energy, int shape, bool pass1 = true,
+            bool pass2 = true) {
+  switch (shape) {
+    case 0: launch_shape<4, 2>(c, a, energy, pass1, pass2); break;
+    case 1: launch_shape<4, 4>(c, a, energy, pass1, pass2); break;
+    case 2: launch_shape<8, 1>(c, a, energy, pass1, pass2); break;
+    case 3: launch_shape<8, 2>(c, a, energy, pass1, pass2); break;
+    default: launch_shape<2, 4>(c, a, energy, pass1, pass2); break;
+  }
+}
+
+// Time every shape of one pass (three launches each, after one warm-up) on
+// the current list and keep the fastest.  Both passes recompute their
+// outputs from their inputs, so the extra launches change nothing.
+int tune_pass(Ctx& c, const EamArgs& a, bool pass1) {
+  hipEvent_t e0, e1;
+  OMDG_HIP(hipEventCreate(&e0));
+  OMDG_HIP(hipEventCreate(&e1));
+  const bool was_timing = c.timing;
+  c.timing = false;
+  int best = 0;
+  float best_ms = 1e30f;
+  for (int v = 0; v < kNumShapes; v++) {
+    launch(c, a, false, v, pass1, !pass1);
+    OMDG_HIP(hipEventRecord(e0, c.stream));
+    for (int r = 0; r < 3; r++) launch(c, a, false, v, pass1, !pass1);
+    OMDG_HIP(hipEventRecord(e1, c.stream));
+    OMDG_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    OMDG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = v;
+    }
+  }
+  c.timing = was_timing;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
 }
 
 }  // namespace
@@ -137,9 +334,13 @@ void eam_forces(Ctx& c, bool energy) {
   a.vir = c.vir32;
   a.err = c.dstat + ST_ERR;
   a.n = c.n;
-  a.phi = tab_view(c, c.tab_index[OMDG_TAB_PAIR][0][0]);
-  a.edens = tab_view(c, c.tab_index[OMDG_TAB_EDENS][0][0]);
+  const int tp = c.tab_index[OMDG_TAB_PAIR][0][0];
+  const int te = c.tab_index[OMDG_TAB_EDENS][0][0];
+  a.phi = tab_view(c, tp);
+  a.edens = tab_view(c, te);
   a.embed = tab_view(c, c.tab_index[OMDG_TAB_EMBED][0][0]);
+  if (!c.eam_pack_ok) build_pack(c, tp, te);
+  a.pack = c.eam_pack;
   // cut radii: table max ranges (team.cpp:78, team.cpp:212)
   const double ce = a.edens.r0d + (double)(a.edens.n - 1) * a.edens.drd;
   const double cp = a.phi.r0d + (double)(a.phi.n - 1) * a.phi.drd;
@@ -148,17 +349,22 @@ void eam_forces(Ctx& c, bool energy) {
   a.hx = (float)c.grid.h[0];
   a.hy = (float)c.grid.h[1];
   a.hz = (float)c.grid.h[2];
-  const int nb = grid_blocks(c.n, kBlock);
-  {
-    TimerScope ts(c, TG_PASS1);
-    if (energy) k_eam_density<true><<<nb, kBlock, 0, c.stream>>>(a);
-    else k_eam_density<false><<<nb, kBlock, 0, c.stream>>>(a);
+  if (c.eam_shape[0] < 0) {
+    // small systems keep the default; large ones time the candidates
+    if (c.n >= 16384) {
+      launch(c, a, false, 0, true, false);  // F'(rho) for the force-pass tuning
+      c.eam_shape[0] = tune_pass(c, a, true);
+      c.eam_shape[1] = tune_pass(c, a, false);
+      if (std::getenv("OMDG_VERBOSE"))
+        std::fprintf(stderr, "[omdg] EAM shapes: density lanes=%d chunk=%d, force lanes=%d chunk=%d\n",
+                     kShapes[c.eam_shape[0]].lanes, kShapes[c.eam_shape[0]].chunk,
+                     kShapes[c.eam_shape[1]].lanes, kShapes[c.eam_shape[1]].chunk);
+    } else {
+      c.eam_shape[0] = c.eam_shape[1] = 0;
+    }
   }
-  {
-    TimerScope ts(c, TG_FORCE);
-    if (energy) k_eam_force<true><<<nb, kBlock, 0, c.stream>>>(a);
-    else k_eam_force<false><<<nb, kBlock, 0, c.stream>>>(a);
-  }
+  launch(c, a, energy, c.eam_shape[0], true, false);
+  launch(c, a, energy, c.eam_shape[1], false, true);
   OMDG_HIP(hipGetLastError());
 }
 

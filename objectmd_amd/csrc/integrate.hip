@@ -29,9 +29,11 @@ struct IntArgs {
   const float4* qb;
   const double* xb;
   int* stat;
+  float* part_max;  // per-block max |disp|^2 and |v|^2
   int n;
   double dt, hdt;
   double mass[kMaxTypes];
+  double hdt_over_m[kMaxTypes];  // FP32-force mode: (dt/2)/m
   int M[3];
   double origin[3], h[3];
   float half_skin2;
@@ -42,7 +44,10 @@ __device__ inline float wave_max(float v) {
   return v;
 }
 
-__device__ inline void block_max_atomic(float a, float b, int* dst_a, int* dst_b) {
+// Block maxima go to a per-block slot with a plain store (thousands of
+// same-address atomics would serialise in L2); k_status_max folds the slots
+// into the status words when the host checks the list.
+__device__ inline void block_max_store(float a, float b, float* part) {
   __shared__ float sa[kBlock / 64], sb[kBlock / 64];
   a = wave_max(a);
   b = wave_max(b);
@@ -58,9 +63,36 @@ __device__ inline void block_max_atomic(float a, float b, int* dst_a, int* dst_b
       ma = fmaxf(ma, sa[k]);
       mb = fmaxf(mb, sb[k]);
     }
+    part[2 * blockIdx.x] = ma;
+    part[2 * blockIdx.x + 1] = mb;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_status_max(const float* part, int nb, int* stat) {
+  float a = 0.f, b = 0.f;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+    a = fmaxf(a, part[2 * k]);
+    b = fmaxf(b, part[2 * k + 1]);
+  }
+  __shared__ float sa[kBlock / 64], sb[kBlock / 64];
+  a = wave_max(a);
+  b = wave_max(b);
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  if (l == 0) {
+    sa[w] = a;
+    sb[w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kBlock / 64; k++) {
+      a = fmaxf(a, sa[k]);
+      b = fmaxf(b, sb[k]);
+    }
+    a = fmaxf(a, sa[0]);
+    b = fmaxf(b, sb[0]);
     // non-negative floats order like their bit patterns
-    atomicMax(dst_a, __float_as_int(ma));
-    atomicMax(dst_b, __float_as_int(mb));
+    stat[ST_DISP2] = __float_as_int(a);
+    stat[ST_VMAX2] = __float_as_int(b);
   }
 }
 
@@ -84,16 +116,21 @@ __global__ void __launch_bounds__(kBlock) k_kick_drift(IntArgs a) {
       ty = a.type[s];
     } else {
       qb = a.qb[s];
-      ty = __float_as_int(qb.w);
+      ty = snap_type(qb.w);
     }
     const double m = a.mass[ty];
+    const double hdt_m = a.hdt_over_m[ty];
     double xs[3];
     for (int d = 0; d < 3; d++) {
       const size_t k = (size_t)d * n + s;
       double x = a.x[k];
       if (move) {
-        const double fd = FP64F ? a.f64[k] : (double)a.f32[k];
-        double v = a.v[k] + a.hdt * fd / m;
+        double v;
+        if (FP64F) {
+          v = a.v[k] + a.hdt * a.f64[k] / m;  // integrator.cpp:236 order
+        } else {
+          v = a.v[k] + (double)a.f32[k] * hdt_m;
+        }
         x = x + a.dt * v;
         a.v[k] = v;
         a.x[k] = x;
@@ -109,8 +146,8 @@ __global__ void __launch_bounds__(kBlock) k_kick_drift(IntArgs a) {
       }
       disp2 = (float)d2;
     } else {
-      const int c = a.cellid[s];
-      const int k0 = c % a.M[0], k1 = (c / a.M[0]) % a.M[1], k2 = c / (a.M[0] * a.M[1]);
+      int k0, k1, k2;
+      snap_cell(qb.w, k0, k1, k2);
       float4 qn;
       qn.x = (float)(xs[0] - (a.origin[0] + (double)k0 * a.h[0]));
       qn.y = (float)(xs[1] - (a.origin[1] + (double)k1 * a.h[1]));
@@ -122,7 +159,7 @@ __global__ void __launch_bounds__(kBlock) k_kick_drift(IntArgs a) {
     }
     if (disp2 > a.half_skin2) atomicOr(a.stat + ST_ERR, ERR_SKIN);
   }
-  block_max_atomic(disp2, v2max, a.stat + ST_DISP2, a.stat + ST_VMAX2);
+  block_max_store(disp2, v2max, a.part_max);
 }
 
 template <bool FP64F, bool FLAGS>
@@ -135,11 +172,11 @@ __global__ void __launch_bounds__(kBlock) k_kick(IntArgs a) {
     if (!(fl & OMDG_FLAG_ACTIVE) || (fl & OMDG_FLAG_FIXED)) return;
   }
   const int n = a.n;
-  const double m = a.mass[a.type[s]];
+  const int ty = a.type[s];
   for (int d = 0; d < 3; d++) {
     const size_t k = (size_t)d * n + s;
-    const double fd = FP64F ? a.f64[k] : (double)a.f32[k];
-    a.v[k] = a.v[k] + a.hdt * fd / m;
+    if (FP64F) a.v[k] = a.v[k] + a.hdt * a.f64[k] / a.mass[ty];  // integrator.cpp:253
+    else a.v[k] = a.v[k] + (double)a.f32[k] * a.hdt_over_m[ty];
   }
 }
 
@@ -156,10 +193,14 @@ IntArgs make_args(Ctx& c, double dt) {
   a.qb = c.qb;
   a.xb = c.xb;
   a.stat = c.dstat;
+  a.part_max = c.dmax;
   a.n = c.n;
   a.dt = dt;
   a.hdt = 0.5 * dt;
-  for (int t = 0; t < kMaxTypes; t++) a.mass[t] = c.mass[t];
+  for (int t = 0; t < kMaxTypes; t++) {
+    a.mass[t] = c.mass[t];
+    a.hdt_over_m[t] = a.hdt / c.mass[t];
+  }
   for (int d = 0; d < 3; d++) {
     a.M[d] = c.grid.M[d];
     a.origin[d] = c.grid.origin[d];
@@ -229,12 +270,25 @@ __global__ void __launch_bounds__(kBlock) k_energy_part(RedArgs a) {
   }
 }
 
-__global__ void k_energy_final(const double* part, int nb, double* out) {
-  if (threadIdx.x != 0) return;
+// one workgroup of kBlock threads sums the per-block partials
+__global__ void __launch_bounds__(kBlock) k_energy_final(const double* part, int nb,
+                                                         double* out) {
   double r[3] = {0, 0, 0};
-  for (int b = 0; b < nb; b++)
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
     for (int q = 0; q < 3; q++) r[q] += part[b * 3 + q];
-  for (int q = 0; q < 3; q++) out[q] = r[q];
+  __shared__ double sh[3][kBlock / 64];
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  for (int q = 0; q < 3; q++) {
+    r[q] = wave_sum(r[q]);
+    if (l == 0) sh[q][w] = r[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 3; q++) {
+      double s = 0;
+      for (int k = 0; k < kBlock / 64; k++) s += sh[q][k];
+      out[q] = s;
+    }
 }
 
 }  // namespace
@@ -265,6 +319,11 @@ void integrate_kick(Ctx& c, double dt) {
   OMDG_HIP(hipGetLastError());
 }
 
+void integrate_status(Ctx& c) {
+  k_status_max<<<1, kBlock, 0, c.stream>>>(c.dmax, grid_blocks(c.n, kBlock), c.dstat);
+  OMDG_HIP(hipGetLastError());
+}
+
 void zero_forces(Ctx& c) {
   OMDG_HIP(hipMemsetAsync(c.f32, 0, sizeof(float) * 3 * (size_t)c.n, c.stream));
   OMDG_HIP(hipMemsetAsync(c.f64, 0, sizeof(double) * 3 * (size_t)c.n, c.stream));
@@ -288,7 +347,7 @@ void reduce_energy(Ctx& c, bool kinetic, bool potential) {
   for (int t = 0; t < kMaxTypes; t++) a.mass[t] = c.mass[t];
   const int nb = c.red_blocks;
   k_energy_part<<<nb, kBlock, 0, c.stream>>>(a);
-  k_energy_final<<<1, 64, 0, c.stream>>>(c.dred, nb, c.dres);
+  k_energy_final<<<1, kBlock, 0, c.stream>>>(c.dred, nb, c.dres);
   OMDG_HIP(hipGetLastError());
   OMDG_HIP(hipMemcpyAsync(c.hres, c.dres, sizeof(double) * 3, hipMemcpyDeviceToHost, c.stream));
   OMDG_HIP(hipStreamSynchronize(c.stream));

@@ -14,8 +14,9 @@
 //   build      one thread per atom scans the 27 stencil cells, tests
 //              r^2 <= rlist^2 in FP64 exactly as the reference (shift the
 //              image first, subtract, sum without FMA) and writes the ELL list
-//              nbr[k * n + i] = slot | stencil-offset << 27; also writes the
-//              FP32 cell-relative snapshot used by the FP32 force kernels.
+//              nbr[nidx(i, k)] = slot | stencil-offset << 27 (ELL in blocks
+//              of four, see nidx); a snapshot kernel first writes the FP32
+//              cell-relative positions used by the FP32 force kernels.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -144,21 +145,34 @@ __global__ void k_scatter(const int* __restrict__ cellid, int n, const int* __re
   perm[slot] = i;
 }
 
-// order each cell by original atom index (insertion sort; cells are small)
-__global__ void k_cell_sort(int* __restrict__ perm, const int* __restrict__ start, int nc,
-                            const int* __restrict__ id) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
-  int b = start[c], e = start[c + 1];
-  for (int i = b + 1; i < e; i++) {
-    int p = perm[i];
-    int key = id[p];
-    int j = i - 1;
-    while (j >= b && id[perm[j]] > key) {
-      perm[j + 1] = perm[j];
-      j--;
+// Order each cell by original atom index, one wavefront per cell: every lane
+// holds one member and its rank is the number of members with a smaller
+// index (indices are distinct).  Cells of more than 64 atoms fall back to
+// an insertion sort by lane 0.
+__global__ void __launch_bounds__(256) k_cell_sort(int* __restrict__ perm,
+                                                   const int* __restrict__ start, int nc,
+                                                   const int* __restrict__ id) {
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x % 64;
+  if (c >= nc) return;  // whole wavefronts leave together (64 | blockDim)
+  const int b = start[c], e = start[c + 1], m = e - b;
+  if (m <= 64) {
+    const int p = lane < m ? perm[b + lane] : 0;
+    const int key = lane < m ? id[p] : 0x7fffffff;
+    int rank = 0;
+    for (int u = 0; u < m; u++) rank += (__shfl(key, u, 64) < key) ? 1 : 0;
+    if (lane < m) perm[b + rank] = p;
+  } else if (lane == 0) {
+    for (int i = b + 1; i < e; i++) {
+      const int p = perm[i];
+      const int key = id[p];
+      int j = i - 1;
+      while (j >= b && id[perm[j]] > key) {
+        perm[j + 1] = perm[j];
+        j--;
+      }
+      perm[j + 1] = p;
     }
-    perm[j + 1] = p;
   }
 }
 
@@ -197,6 +211,10 @@ struct BuildArgs {
   int maxnbr;
   int has_flags;
   double rl2;
+  // FP32 decision band around rlist^2: the snapshot distance differs from
+  // the FP64 one by less than 4e-6 rlist^2 (|q| < 2 rlist, three FP32
+  // roundings per component), the band is +-3e-5 rlist^2
+  float rl2_lo, rl2_hi;
   GridDev g;
 };
 
@@ -204,28 +222,47 @@ __device__ inline double shifted(double x, int s, double L) {
   return s == 0 ? x : (s < 0 ? x + (-L) : x + L);
 }
 
-__global__ void __launch_bounds__(256) k_build(BuildArgs a) {
-#pragma clang fp contract(off)
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+// FP32 snapshot relative to the cell origin, the build copy of it and the
+// FP64 build positions.
+__global__ void __launch_bounds__(256) k_snapshot(BuildArgs a) {
+  const int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= a.n) return;
   const int n = a.n;
   const double xi = a.x[s], yi = a.x[(size_t)n + s], zi = a.x[2 * (size_t)n + s];
   const int c = a.cellid[s];
   const int k0 = c % a.g.M[0], k1 = (c / a.g.M[0]) % a.g.M[1], k2 = c / (a.g.M[0] * a.g.M[1]);
-  // FP32 snapshot relative to the cell origin
-  const double ox = a.g.origin[0] + (double)k0 * a.g.h[0];
-  const double oy = a.g.origin[1] + (double)k1 * a.g.h[1];
-  const double oz = a.g.origin[2] + (double)k2 * a.g.h[2];
   float4 qv;
-  qv.x = (float)(xi - ox);
-  qv.y = (float)(yi - oy);
-  qv.z = (float)(zi - oz);
-  qv.w = __int_as_float(a.type[s]);
+  qv.x = (float)(xi - (a.g.origin[0] + (double)k0 * a.g.h[0]));
+  qv.y = (float)(yi - (a.g.origin[1] + (double)k1 * a.g.h[1]));
+  qv.z = (float)(zi - (a.g.origin[2] + (double)k2 * a.g.h[2]));
+  qv.w = snap_word(a.type[s], k0, k1, k2);
   a.q[s] = qv;
   a.qb[s] = qv;
   a.xb[s] = xi;
   a.xb[(size_t)n + s] = yi;
   a.xb[2 * (size_t)n + s] = zi;
+}
+
+// Membership is decided by the reference's FP64 test; the FP32 snapshot
+// distance (error below 1e-5 of rlist^2, see the band) settles every
+// candidate outside a thin band around rlist without touching the FP64
+// coordinates.
+__global__ void __launch_bounds__(256) k_build(BuildArgs a) {
+#pragma clang fp contract(off)
+  __shared__ float4 offs[27];
+  for (int i = threadIdx.x; i < 27; i += blockDim.x) {
+    const int ox = i % 3 - 1, oy = (i / 3) % 3 - 1, oz = i / 9 - 1;
+    offs[i] = make_float4((float)ox * (float)a.g.h[0], (float)oy * (float)a.g.h[1],
+                          (float)oz * (float)a.g.h[2], 0.f);
+  }
+  __syncthreads();
+  const int s = xcd_block() * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  const int n = a.n;
+  const double xi = a.x[s], yi = a.x[(size_t)n + s], zi = a.x[2 * (size_t)n + s];
+  const float4 qi = a.q[s];
+  int k0, k1, k2;
+  snap_cell(qi.w, k0, k1, k2);
   if (a.has_flags && !(a.flag[s] & OMDG_FLAG_ACTIVE)) {
     a.nnbr[s] = 0;
     return;
@@ -259,16 +296,28 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a) {
         const int cc = kx + a.g.M[0] * (ky + a.g.M[1] * kz);
         const uint32_t img = (uint32_t)((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1));
         const bool noshift = (sx | sy | sz) == 0;
+        const float4 o = offs[img];
         const int e = a.start[cc + 1];
         for (int t = a.start[cc]; t < e; t++) {
           if (t == s && noshift) continue;
           if (a.has_flags && !(a.flag[t] & OMDG_FLAG_ACTIVE)) continue;
-          const double ddx = xi - shifted(a.x[t], sx, a.g.L[0]);
-          const double ddy = yi - shifted(a.x[(size_t)n + t], sy, a.g.L[1]);
-          const double ddz = zi - shifted(a.x[2 * (size_t)n + t], sz, a.g.L[2]);
-          const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-          if (r2 <= a.rl2) {
-            if (cnt < a.maxnbr) a.nbr[(size_t)cnt * n + s] = (uint32_t)t | (img << kImgShift);
+          const float4 qt = a.q[t];
+          const float fx = qi.x - qt.x - o.x, fy = qi.y - qt.y - o.y, fz = qi.z - qt.z - o.z;
+          const float r2f = fx * fx + fy * fy + fz * fz;
+          bool in;
+          if (r2f < a.rl2_lo) {
+            in = true;
+          } else if (r2f > a.rl2_hi) {
+            in = false;
+          } else {
+            const double ddx = xi - shifted(a.x[t], sx, a.g.L[0]);
+            const double ddy = yi - shifted(a.x[(size_t)n + t], sy, a.g.L[1]);
+            const double ddz = zi - shifted(a.x[2 * (size_t)n + t], sz, a.g.L[2]);
+            const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            in = r2 <= a.rl2;
+          }
+          if (in) {
+            if (cnt < a.maxnbr) a.nbr[nidx(s, cnt, n)] = (uint32_t)t | (img << kImgShift);
             cnt++;
           }
         }
@@ -355,7 +404,8 @@ void nbr_build(Ctx& c) {
   k_scan<<<1, 1024, 0, c.stream>>>(c.cell_fill, c.cell_start, (int)nc);
   k_scatter<<<grid_blocks(n, B), B, 0, c.stream>>>(c.cellid, n, c.cell_start, c.cell_fill,
                                                    c.perm);
-  k_cell_sort<<<grid_blocks(nc, B), B, 0, c.stream>>>(c.perm, c.cell_start, (int)nc, c.id[src]);
+  k_cell_sort<<<grid_blocks(nc * 64, B), B, 0, c.stream>>>(c.perm, c.cell_start, (int)nc,
+                                                            c.id[src]);
   int* cell_dst = c.cellid2;
   k_permute<<<grid_blocks(n, B), B, 0, c.stream>>>(
       n, c.perm, c.x[src], c.v[src], c.type[src], c.flag[src], c.id[src], c.cellid, c.x[dst],
@@ -380,7 +430,10 @@ void nbr_build(Ctx& c) {
   a.maxnbr = c.maxnbr;
   a.has_flags = c.has_flags ? 1 : 0;
   a.rl2 = rl * rl;
+  a.rl2_lo = (float)(a.rl2 * (1.0 - 3e-5));
+  a.rl2_hi = (float)(a.rl2 * (1.0 + 3e-5));
   a.g = gd;
+  k_snapshot<<<grid_blocks(n, 256), 256, 0, c.stream>>>(a);
   k_build<<<grid_blocks(n, 256), 256, 0, c.stream>>>(a);
   OMDG_HIP(hipGetLastError());
   c.forces_valid = false;

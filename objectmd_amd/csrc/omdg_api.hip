@@ -171,6 +171,7 @@ static void free_atoms(Ctx& c) {
   dfree(c.fp);
   dfree(c.emb);
   dfree(c.perm);
+  dfree(c.dmax);
   dfree(c.dred);
   c.cap_atoms = 0;
 }
@@ -200,6 +201,7 @@ static void alloc_atoms(Ctx& c, int n) {
   dalloc(c.fp, n);
   dalloc(c.emb, n);
   dalloc(c.perm, n);
+  dalloc(c.dmax, 2 * (size_t)grid_blocks(n, 256));
   c.red_blocks = std::min(1024, grid_blocks(n, 256));
   dalloc(c.dred, 8 * (size_t)c.red_blocks);
   c.cap_atoms = n;
@@ -218,6 +220,7 @@ static void upload_tables(Ctx& c) {
                        hipMemcpyHostToDevice));
   }
   c.tables_dirty = false;
+  c.eam_pack_ok = false;
 }
 
 static void check_ready(Ctx& c) {
@@ -413,6 +416,7 @@ int omdg_destroy(omdg_t h) {
   dfree(c.cell_start);
   dfree(c.cell_fill);
   dfree(c.tab32);
+  dfree(c.eam_pack);
   dfree(c.tab64);
   dfree(c.dstat);
   dfree(c.dres);
@@ -491,6 +495,7 @@ int omdg_upload_atoms(omdg_t h, int n, const double* x3, const double* v3, const
     OMDG_HIP(hipMemset(c.dstat, 0, sizeof(int) * ST_COUNT));
     c.list_valid = false;
     c.forces_valid = false;
+    c.eam_shape[0] = c.eam_shape[1] = -1;
   });
 }
 
@@ -701,8 +706,9 @@ int omdg_set_neighbor(omdg_t h, double skin, int check_every, int maxnbr) {
     Ctx& c = h->c;
     c.skin = skin;
     c.check_every = check_every;
-    c.maxnbr_req = maxnbr;
-    if (maxnbr > 0) c.maxnbr = maxnbr;
+    // the list is stored in blocks of four entries (nidx)
+    c.maxnbr_req = (maxnbr + 3) / 4 * 4;
+    if (maxnbr > 0) c.maxnbr = c.maxnbr_req;
     c.list_valid = false;
   });
 }
@@ -744,7 +750,7 @@ int omdg_neighbor_export(omdg_t h, int maxn, int* cnt, int* code) {
       std::vector<int> row;
       row.reserve(nn[s]);
       for (int e = 0; e < nn[s]; e++) {
-        uint32_t w = lst[(size_t)e * n + s];
+        uint32_t w = lst[nidx(s, e, n)];
         int t = (int)(w & kSlotMask), img = (int)(w >> kImgShift);
         int o[3] = {img % 3 - 1, (img / 3) % 3 - 1, img / 9 - 1};
         int sh[3];
@@ -820,6 +826,7 @@ int omdg_step(omdg_t h, int nsteps, double dt, double* ekin, double* epot) {
         integrate_kick_drift(c, dt);
       }
       if (--until_check == 0) {
+        integrate_status(c);
         read_status(c);
         raise_device_errors(c);
         double disp = std::sqrt((double)bits_to_float(c.hstat[ST_DISP2]));

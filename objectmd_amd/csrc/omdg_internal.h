@@ -107,6 +107,7 @@ struct Ctx {
   float* rho = nullptr;
   float* fp = nullptr;  // F'(rho)
   float* emb = nullptr; // F(rho)
+  float* dmax = nullptr;    // per-block maxima of the drift (2 per block)
   float* bond_b = nullptr;  // Tersoff per-slot bond order (ELL like nbr)
   float* bond_p = nullptr;  // Tersoff per-slot zeta prefactor
   size_t cap_bond = 0;
@@ -142,6 +143,11 @@ struct Ctx {
   float* tab32 = nullptr;
   double* tab64 = nullptr;
   bool tables_dirty = true;
+  // EAM: PAIR and EDENS coefficients interleaved per interval (2 x float4)
+  // when both tables share one grid
+  float4* eam_pack = nullptr;
+  bool eam_pack_ok = false;
+  int eam_shape[2] = {-1, -1};  // tuned (lanes, chunk) shape of the two passes
   int tab_index[3][kMaxTypes][kMaxTypes];  // kind -> (ta,tb) -> table id
   SwParams sw{};
   TersoffParams ters{};
@@ -168,6 +174,40 @@ struct TimerScope {
   ~TimerScope();
 };
 
+// Neighbour-list layout: ELL in blocks of 4 entries, entry k of atom s at
+// ((k / 4) * n + s) * 4 + k % 4.  One lane per atom reads 16 B per block;
+// four lanes per atom (lane j takes entries k = j mod 4) read 64 contiguous
+// words per wavefront.
+__host__ __device__ inline size_t nidx(int s, int k, int n) {
+  return ((size_t)(k >> 2) * (size_t)n + (size_t)s) * 4 + (size_t)(k & 3);
+}
+
+// The w word of an FP32 snapshot holds the atom type (2 bits) and the cell
+// coordinates of the last build (10 bits each); it is only ever moved and
+// reinterpreted, never used in float arithmetic.
+__device__ inline float snap_word(int type, int k0, int k1, int k2) {
+  return __int_as_float(type | (k0 << 2) | (k1 << 12) | (k2 << 22));
+}
+__device__ inline int snap_type(float w) { return __float_as_int(w) & 3; }
+
+// XCD-aware workgroup order.  The dispatcher hands workgroup b to XCD b % 8;
+// each XCD has its own L2.  Renumbering gives XCD x one contiguous range of
+// logical blocks, so with atoms in cell order the neighbour gathers of one
+// XCD stay inside one compact region of the system and its L2.
+constexpr int kNumXcd = 8;
+__device__ inline int xcd_block() {
+  const int b = blockIdx.x, nb = gridDim.x;
+  const int q = nb / kNumXcd, r = nb % kNumXcd;
+  const int x = b % kNumXcd, i = b / kNumXcd;
+  return x * q + (x < r ? x : r) + i;
+}
+__device__ inline void snap_cell(float w, int& k0, int& k1, int& k2) {
+  const int b = __float_as_int(w);
+  k0 = (b >> 2) & 1023;
+  k1 = (b >> 12) & 1023;
+  k2 = (b >> 22) & 1023;
+}
+
 // ---- launchers (defined in the .hip files) --------------------------------
 void nbr_build(Ctx& c);
 void eam_forces(Ctx& c, bool energy);
@@ -176,6 +216,7 @@ void sw_forces(Ctx& c, bool energy);
 void tersoff_forces(Ctx& c, bool energy);
 void integrate_kick_drift(Ctx& c, double dt);
 void integrate_kick(Ctx& c, double dt);
+void integrate_status(Ctx& c);  // fold the drift's block maxima into dstat
 void reduce_energy(Ctx& c, bool kinetic, bool potential);
 void zero_forces(Ctx& c);
 

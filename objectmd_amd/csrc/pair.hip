@@ -49,7 +49,7 @@ __device__ inline double shifted(double x, int s, double L) {
 
 template <bool ENERGY>
 __global__ void __launch_bounds__(kBlock) k_pair64(PairArgs a) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= a.n) return;
   const int n = a.n;
   const double xi = a.x[s], yi = a.x[(size_t)n + s], zi = a.x[2 * (size_t)n + s];
@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(kBlock) k_pair64(PairArgs a) {
   const int nn = a.nnbr[s];
   double fx = 0, fy = 0, fz = 0, pot = 0, vir = 0;
   for (int e = 0; e < nn; e++) {
-    const uint32_t w = a.nbr[(size_t)e * n + s];
+    const uint32_t w = a.nbr[nidx(s, e, n)];
     const int t = (int)(w & kSlotMask);
     const int img = (int)(w >> kImgShift);
     const int o[3] = {img % 3 - 1, (img / 3) % 3 - 1, img / 9 - 1};
@@ -112,18 +112,18 @@ __global__ void __launch_bounds__(kBlock) k_pair32(PairArgs a) {
   __shared__ float4 offs[27];
   load_offsets(offs, a.hx, a.hy, a.hz);
   __syncthreads();
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= a.n) return;
   const int n = a.n;
   const float4 qi = a.q[s];
-  const int ti = __float_as_int(qi.w);
+  const int ti = snap_type(qi.w);
   const int nn = a.nnbr[s];
   Kahan fx, fy, fz;
   float pot = 0.f, vir = 0.f;
   for (int e = 0; e < nn; e++) {
-    const uint32_t w = a.nbr[(size_t)e * n + s];
+    const uint32_t w = a.nbr[nidx(s, e, n)];
     const float4 qj = a.q[w & kSlotMask];
-    const int slot = ti * kMaxTypes + __float_as_int(qj.w);
+    const int slot = ti * kMaxTypes + snap_type(qj.w);
     const int tb = a.tab[slot];
     if (tb < 0) continue;
     const float4 o = offs[w >> kImgShift];

@@ -118,6 +118,25 @@ __device__ inline bool spline_value64(const TabView& t, double r, double& val, i
   return true;
 }
 
+// Branch-free form of spline_locate32 for hot loops: range errors are OR'd
+// into `err` (one atomic per lane at the end of the kernel), and r is
+// clamped so the returned segment is always inside the table.
+__device__ inline int spline_locate32_nb(const TabView& t, float r, float& dx, int& err) {
+  const float rval = r - t.r0;
+  float qf = rval * t.inv_dr;
+  err |= !(qf < (float)t.n) ? ERR_RANGE_HI : 0;  // also NaN
+  err |= (qf <= -2.0f) ? ERR_RANGE_LO : 0;
+  qf = fminf(fmaxf(qf, -1.5f), (float)t.n - 0.5f);
+  int x = (int)qf;  // truncation, in [-1, n-1]
+  float d = rval - (float)x * t.dr;
+  const bool top = (x == t.n - 1);
+  d = top ? d + t.dr : d;
+  x = top ? x - 1 : x;
+  x = x < 0 ? 0 : x;
+  dx = d;
+  return x;
+}
+
 // Compensated (Kahan) accumulation of FP32 terms.
 struct Kahan {
   float s = 0.f, c = 0.f;

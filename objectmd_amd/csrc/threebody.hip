@@ -127,11 +127,11 @@ __global__ void __launch_bounds__(kBlock) k_sw(TBArgs a) {
   __shared__ float4 offs[27];
   load_offsets(offs, a.hx, a.hy, a.hz);
   __syncthreads();
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = xcd_block() * blockDim.x + threadIdx.x;
   if (l >= a.n) return;
   const int n = a.n;
   const float4 ql = a.q[l];
-  const int tl = __float_as_int(ql.w);
+  const int tl = snap_type(ql.w);
   const bool l_sw = a.sw_type < 0 || tl == a.sw_type;
   Kahan fx, fy, fz;
   float pot = 0.f, vir = 0.f;
@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(kBlock) k_sw(TBArgs a) {
   const int cnt = a.nnbr[l];
   // two-body terms and the neighbours inside the SW cut
   for (int k = 0; k < cnt; k++) {
-    const uint32_t e = a.nbr[(size_t)k * n + l];
+    const uint32_t e = a.nbr[nidx(l, k, n)];
     const float4 qj = a.q[e & kSlotMask];
     const float3 d = disp(ql, qj, offs[e >> kImgShift]);  // x_j - x_l
     const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(kBlock) k_sw(TBArgs a) {
       atomicOr(a.err, ERR_OVERLAP);
       continue;
     }
-    const int tj = __float_as_int(qj.w);
+    const int tj = snap_type(qj.w);
     const bool pair_sw = l_sw && (a.sw_type < 0 || tj == a.sw_type);
     if (pair_sw) {
       if (r2 < a.rc2) {
@@ -206,10 +206,10 @@ __global__ void __launch_bounds__(kBlock) k_sw(TBArgs a) {
     const float r1 = nr[u].r;
     const int cc = a.nnbr[c];
     for (int k = 0; k < cc; k++) {
-      const uint32_t e = a.nbr[(size_t)k * n + c];
+      const uint32_t e = a.nbr[nidx(c, k, n)];
       const int t = (int)(e & kSlotMask);
       const float4 qk = a.q[t];
-      if (a.sw_type >= 0 && __float_as_int(qk.w) != a.sw_type) continue;
+      if (a.sw_type >= 0 && snap_type(qk.w) != a.sw_type) continue;
       const float3 d = disp(qc, qk, offs[e >> kImgShift]);  // x_k - x_c
       const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
       if (!(r2 < a.rc2)) continue;
@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_bond(TBArgs a) {
   __shared__ float4 offs[27];
   load_offsets(offs, a.hx, a.hy, a.hz);
   __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const int n = a.n;
   const TersoffParams& p = a.ts;
@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_bond(TBArgs a) {
   int slot[kMaxNear];
   int nn = 0;
   for (int k = 0; k < cnt; k++) {
-    const uint32_t e = a.nbr[(size_t)k * n + i];
+    const uint32_t e = a.nbr[nidx(i, k, n)];
     const float3 d = disp(qi, a.q[e & kSlotMask], offs[e >> kImgShift]);
     const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
     if (r2 < 1e-6f) atomicOr(a.err, ERR_OVERLAP);
@@ -362,7 +362,7 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_force(TBArgs a) {
   __shared__ float4 offs[27];
   load_offsets(offs, a.hx, a.hy, a.hz);
   __syncthreads();
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = xcd_block() * blockDim.x + threadIdx.x;
   if (l >= a.n) return;
   const int n = a.n;
   const TersoffParams& p = a.ts;
@@ -373,7 +373,7 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_force(TBArgs a) {
   int slot[kMaxNear];
   int nn = 0;
   for (int k = 0; k < cnt && nn < kMaxNear; k++) {
-    const uint32_t e = a.nbr[(size_t)k * n + l];
+    const uint32_t e = a.nbr[nidx(l, k, n)];
     const float3 d = disp(ql, a.q[e & kSlotMask], offs[e >> kImgShift]);
     const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
     if (r2 < a.rc2 && r2 >= 1e-6f) {
@@ -429,7 +429,7 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_force(TBArgs a) {
     int ms[kMaxNear];
     int mm = 0, self = -1;
     for (int k = 0; k < cj && mm < kMaxNear; k++) {
-      const uint32_t e = a.nbr[(size_t)k * n + j];
+      const uint32_t e = a.nbr[nidx(j, k, n)];
       const int t = (int)(e & kSlotMask);
       const float3 d = disp(qj, a.q[t], offs[e >> kImgShift]);  // x_t - x_j
       const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
