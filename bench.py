@@ -133,6 +133,11 @@ def main():
     if args.copy_probe_mb:
         copy_probe(args.copy_probe_mb)
 
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_eam256k.json")
+    if os.path.exists(tpath) and args.cells == 40:
+        with open(tpath) as fl:
+            traffic = json.load(fl).get("traffic_bytes_per_launch")
     force_ms, force_launches = times["force"]
     avg_force_s = force_ms / max(force_launches, 1) / 1e3
     achieved = FORCE_PASS_BYTES * n / avg_force_s / 1e9 if avg_force_s > 0 else 0.0
@@ -197,7 +202,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
         },
         "cpu_baseline": cpu,
     }

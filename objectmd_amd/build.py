@@ -59,6 +59,18 @@ def build_host_api():
           "-I", os.path.join(host, "include"), "-I", os.path.join(ROOT, "include"),
           *srcs, "-o", tmp, "-L", PKG, "-lomdg", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, out)
+    # example programs written against the Object-MD API
+    bindir = os.path.join(host, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    exdir = os.path.join(host, "examples")
+    for f in sorted(os.listdir(exdir)):
+        if not f.endswith(".cpp"):
+            continue
+        exe = os.path.join(bindir, f[:-4])
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(host, "include"),
+              "-I", os.path.join(ROOT, "include"), os.path.join(exdir, f), "-o", exe + ".tmp",
+              "-L", PKG, "-lomd", "-lomdg", "-Wl,-rpath,$ORIGIN/../.."])
+        os.replace(exe + ".tmp", exe)
     return out
 
 

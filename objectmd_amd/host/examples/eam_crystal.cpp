@@ -1,0 +1,35 @@
+// A user program in the style of Object-MD's test/example/example.cpp:
+// fcc crystal, tabulated EAM, Verlet list, energy monitor.  Material and
+// table names, sizes and the step come from the parameter file:
+//   material <def.* name>  eam <eam.* name>  cells <n>  temperature <K>
+#include <omd.h>
+
+class MyMDClass : public MDSystemGrid {
+  void CreateSystem() {
+    const int cells = param.exist("cells") ? param.int_value("cells") : 8;
+    AddAtom(new FCC("100", 2 * cells, 2 * cells, 2 * cells, param.string_value("material")))
+        ->Create()
+        ->SetTemperature(param.exist("temperature") ? param.double_value("temperature") : 100.0)
+        ->SetName("Crystal");
+  }
+
+  void CreateGadget() {
+    SetIntegrator(new MDIntegrator);
+    AddForce(new TForceEAM(param.string_value("eam")));
+    AddModify(new VerletList);
+    AddDetector(new SysMonitor("md.out"));
+  }
+
+  void SystemSetting() { PBoundary = PERIODIC; }
+
+  void BeforeRun() {
+    DumpAtoms("init.dat");
+    PrintInfo("info.out");
+  }
+};
+
+int main(int argc, char* argv[]) {
+  MyMDClass TheSim;
+  TheSim.SetArgument(argc, argv);
+  return TheSim.Run();
+}

@@ -1,0 +1,40 @@
+// Lennard-Jones argon in FP64 (the plain pair method): fcc crystal, pair
+// table pair.<name>, periodic in x, y and z; InlineFunction reads atoms
+// every step, as the reference example does.
+#include <omd.h>
+
+std::ofstream trjfl;
+
+class LJSim : public MDSystemGrid {
+  void CreateSystem() {
+    const int cells = param.exist("cells") ? param.int_value("cells") : 10;
+    AddAtom(new FCC("100", 2 * cells, 2 * cells, 2 * cells, param.string_value("material")))
+        ->Create()
+        ->SetTemperature(param.exist("temperature") ? param.double_value("temperature") : 40.0)
+        ->SetName("Argon");
+  }
+
+  void CreateGadget() {
+    SetIntegrator(new MDIntegrator);
+    AddForce(new TForcePair(param.string_value("pair")));
+    AddModify(new VerletList);
+    AddDetector(new SysMonitor("md.out"));
+  }
+
+  void SystemSetting() { PBoundary = PERIODIC; }
+
+  void InlineFunction() {
+    trjfl << Atoms(0).x << " " << Atoms(0).y << " " << Atoms(0).z << "\n";
+  }
+
+  void BeforeRun() { DumpAtoms("init.dat"); }
+};
+
+int main(int argc, char* argv[]) {
+  LJSim TheSim;
+  trjfl.open("traj.dat");
+  TheSim.SetArgument(argc, argv);
+  int rc = TheSim.Run();
+  trjfl.close();
+  return rc;
+}

@@ -1,0 +1,31 @@
+// Diamond silicon with the Tersoff potential, free (001) surfaces in z as in
+// the 1,000,000-atom configuration: tersoff.<material> gives the parameters.
+#include <omd.h>
+
+class SiSim : public MDSystemGrid {
+  void CreateSystem() {
+    const int cells = param.exist("cells") ? param.int_value("cells") : 6;
+    AddAtom(new Diamond("100", 4 * (cells - 1), 4 * (cells - 1), 4 * (cells - 1),
+                        param.string_value("material")))
+        ->Create()
+        ->SetTemperature(param.exist("temperature") ? param.double_value("temperature") : 500.0)
+        ->SetName("Silicon");
+  }
+
+  void CreateGadget() {
+    SetIntegrator(new MDIntegrator);
+    AddForce(new Tersoff(param.string_value("tersoff")));
+    AddModify(new VerletListFull);
+    AddDetector(new SysMonitor("md.out"));
+  }
+
+  void SystemSetting() { PBoundary = PERIODIC_X | PERIODIC_Y; }
+
+  void BeforeRun() { DumpAtoms("init.dat"); }
+};
+
+int main(int argc, char* argv[]) {
+  SiSim TheSim;
+  TheSim.SetArgument(argc, argv);
+  return TheSim.Run();
+}

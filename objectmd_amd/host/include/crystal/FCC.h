@@ -1,0 +1,31 @@
+// Crystal builders with the reference's atom order and box
+// (src/gadget/FCC.cpp:142-242, include/crystal/Diamond.h:17-58,
+// include/crystal/Crystalline.h:53-74).
+#ifndef OMD_CRYSTAL_FCC_H
+#define OMD_CRYSTAL_FCC_H
+#include <omd/system.h>
+namespace omd {
+
+class Crystalline : public AtomContainer {
+ protected:
+  string orientation;
+  int xml, yml, zml;
+  double XMLDist = -1, YMLDist = -1, ZMLDist = -1;
+
+ public:
+  double lattice_constant = 0;
+  Crystalline(const string& orien, int x, int y, int z, const string& mat);
+  SysBox& CalcBox() override;  // bounding box widened by half a monolayer
+};
+
+class FCC : public Crystalline {
+ public:
+  FCC(const string& orientation, int XMLayer, int YMLayer, int ZMLayer, const string& mat_file)
+      : Crystalline(orientation, XMLayer, YMLayer, ZMLayer, mat_file) {
+    set_name("FCC");
+  }
+  AtomContainer* Create() override;
+};
+
+}  // namespace omd
+#endif

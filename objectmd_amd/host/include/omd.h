@@ -1,0 +1,17 @@
+// Umbrella header of the Object-MD host API (include/omd.h of the reference).
+#ifndef OMD_H
+#define OMD_H
+#include <omd/system.h>
+#include <crystal/FCC.h>
+#include <crystal/Diamond.h>
+#include <crystal/Projectile.h>
+#include <modify/VerletList.h>
+#include <modify/VerletListFull.h>
+#include <potential/team.h>
+#include <potential/tpair.h>
+#include <potential/sw.h>
+#include <potential/tersoff.h>
+#include <detect/SysMonitor.h>
+using namespace omd;
+using std::ofstream;
+#endif

@@ -1,0 +1,430 @@
+// Object-MD host API: containers, builders, parameters and the main loop
+// over the device engine (C ABI of omdg.h).
+#include <omd.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <sstream>
+
+namespace omd {
+
+// ---------------------------------------------------------------- tools ----
+[[noreturn]] void die(const string& msg) { throw msg; }
+
+void mdassert(bool cond, const string& msg) {
+  if (!cond) die(msg);
+}
+
+string as_string(double v) {
+  std::ostringstream s;
+  s << v;
+  return s.str();
+}
+
+string search_path(const string& env, const string& file) {
+  // $OMD_TABLE names a directory (or a colon-separated list); the current
+  // directory is searched last (src/omd/base.cpp:36-45 resolves the same
+  // variable)
+  const char* e = std::getenv(env.c_str() + (env[0] == '$' ? 1 : 0));
+  vector<string> dirs;
+  if (e) {
+    std::stringstream ss(e);
+    string d;
+    while (std::getline(ss, d, ':'))
+      if (!d.empty()) dirs.push_back(d);
+  }
+  dirs.push_back(".");
+  for (const string& d : dirs) {
+    string p = d + "/" + file;
+    std::ifstream f(p);
+    if (f.good()) return p;
+  }
+  return file;
+}
+
+// ------------------------------------------------------------ parameters ---
+bool ParamHandler::read(const string& filename) {
+  std::ifstream fl(filename);
+  if (!fl.good()) return false;
+  string line;
+  while (std::getline(fl, line)) {
+    const size_t h = line.find('#');
+    if (h != string::npos) line = line.substr(0, h);
+    append(line);
+  }
+  return true;
+}
+
+void ParamHandler::append(const string& line) {
+  std::istringstream s(line);
+  string t;
+  while (s >> t) spar.push_back(t);
+}
+
+int ParamHandler::index_of(const string& key) const {
+  for (int i = (int)spar.size() - 1; i >= 0; i--)  // later values override
+    if (spar[i] == key) return i;
+  return -1;
+}
+
+bool ParamHandler::exist(const string& key) const { return index_of(key) >= 0; }
+
+string ParamHandler::string_value(const string& key, int idx) const {
+  const int i = index_of(key);
+  if (i < 0 || i + 1 + idx >= (int)spar.size()) die("parameter " + key + " doesn't exist");
+  return spar[i + 1 + idx];
+}
+
+double ParamHandler::double_value(const string& key, int idx) const {
+  return std::stod(string_value(key, idx));
+}
+
+int ParamHandler::int_value(const string& key, int idx) const {
+  return std::stoi(string_value(key, idx));
+}
+
+bool ParamHandler::peek(const string& key, double& v) const {
+  if (!exist(key)) return false;
+  v = double_value(key);
+  return true;
+}
+bool ParamHandler::peek(const string& key, int& v) const {
+  if (!exist(key)) return false;
+  v = int_value(key);
+  return true;
+}
+bool ParamHandler::peek(const string& key, string& v) const {
+  if (!exist(key)) return false;
+  v = string_value(key);
+  return true;
+}
+bool ParamHandler::peek(const string& key, bool& v) const {
+  if (!exist(key)) return false;
+  const string s = string_value(key);
+  v = (s == "true" || s == "yes" || s == "1");
+  return true;
+}
+
+// --------------------------------------------------------------- gadgets ---
+void MDGadget::Init(MDSystem* ws) {
+  System = ws;
+  SysParam = &ws->param;
+  ReadParameter();
+}
+
+void MDIterator::ReadParameter() {
+  SysParam->peek("verlet.radius", CutRadius);
+  SysParam->peek("verlet.radtole", RadiusTolerance);
+}
+
+MDIntegrator::~MDIntegrator() {
+  for (ForceKernel* f : ActForces) delete f;
+}
+
+void MDIntegrator::Init(MDSystem* ws) {
+  MDGadget::Init(ws);
+  mdassert(!ActForces.empty(), "no defined force kernel");
+  ws->param.peek("time.step", TimeStep);
+  if (TimeStep <= 0.0) TimeStep = 0.001;  // integrator.cpp:116-122
+  for (ForceKernel* f : ActForces) f->Init(ws);
+}
+
+int ForceKernel::Lower(omdg_t) {
+  die("no device implementation for force kernel " + get_name());
+}
+
+// ------------------------------------------------------------ containers ---
+AtomContainer::AtomContainer(const string& material_file) : material(material_file) {
+  if (material.empty()) return;
+  // def.<material>: mass (amu) and atomic number (container.cpp:85-100)
+  const string path = search_path("$OMD_TABLE", "def." + material);
+  if (param.read(path)) {
+    if (param.exist("mass")) M = param.double_value("mass") / NORMAL_AMU;
+    if (param.exist("number")) Z = param.double_value("number");
+    if (param.exist("element")) set_name(param.string_value("element"));
+  } else {
+    die("can not find material file def." + material);
+  }
+}
+
+void AtomContainer::Allocate(int n) {
+  atoms.assign(n, Atom());
+  for (auto& a : atoms) a.tid = id;
+}
+
+AtomContainer* AtomContainer::SetVelocity(double vx, double vy, double vz) {
+  if (!created) Create();
+  for (auto& a : atoms) {
+    a.vx = vx;
+    a.vy = vy;
+    a.vz = vz;
+  }
+  return this;
+}
+
+// Equal kinetic energy per atom in random directions, then zero momentum and
+// rescale (container.cpp:683-754).  The reference seeds with the process id;
+// here the seed is random.seed (default 12345) so runs repeat.
+AtomContainer* AtomContainer::SetKineticEnergy(double ek) {
+  if (!created) Create();
+  const int n = GetNAtom();
+  if (ek <= 0.0 || n == 0) {
+    for (auto& a : atoms) a.vx = a.vy = a.vz = 0.0;
+    return this;
+  }
+  int seed = 12345;
+  if (System) System->param.peek("random.seed", seed);
+  std::mt19937_64 gen((uint64_t)seed + (uint64_t)id * 7919u);
+  std::uniform_real_distribution<double> U(0.0, 1.0);
+  double sv[3] = {0, 0, 0};
+  for (auto& a : atoms) {
+    const double v = std::sqrt(2.0 * ek / M);
+    const double sx = U(gen) * v;
+    const double sy = U(gen) * (v - sx);
+    const double sz = v - (sx + sy);
+    a.vx = (U(gen) > 0.5) ? -std::sqrt(sx) : std::sqrt(sx);
+    a.vy = (U(gen) > 0.5) ? -std::sqrt(sy) : std::sqrt(sy);
+    a.vz = (U(gen) > 0.5) ? -std::sqrt(sz) : std::sqrt(sz);
+    sv[0] += a.vx;
+    sv[1] += a.vy;
+    sv[2] += a.vz;
+  }
+  for (double& s : sv) s /= (double)n;
+  double check_ek = 0.0;
+  for (auto& a : atoms) {
+    a.vx -= sv[0];
+    a.vy -= sv[1];
+    a.vz -= sv[2];
+    check_ek += 0.5 * M * (a.vx * a.vx + a.vy * a.vy + a.vz * a.vz);
+  }
+  const double fact = std::sqrt(ek * (double)n / check_ek);
+  for (auto& a : atoms) {
+    a.vx *= fact;
+    a.vy *= fact;
+    a.vz *= fact;
+  }
+  return this;
+}
+
+// 3 kT per atom: twice equipartition, as the reference (container.cpp:752-754)
+AtomContainer* AtomContainer::SetTemperature(double temperature) {
+  return SetKineticEnergy(3.0 * temperature * MD_BOLTZMANN);
+}
+
+// Text import with an optional "#$ Fields x y z vx vy vz ..." header
+// (container.cpp:290-404); without a header the columns are x y z.
+AtomContainer* AtomContainer::Import(const string& filename) {
+  std::ifstream fl(filename);
+  if (!fl.good()) die("can not open import file " + filename);
+  vector<string> fields = {"x", "y", "z"};
+  string line;
+  atoms.clear();
+  while (std::getline(fl, line)) {
+    std::istringstream s(line);
+    string t;
+    if (!(s >> t)) continue;
+    if (t == "#$") {
+      string key;
+      s >> key;
+      if (key == "Fields") {
+        fields.clear();
+        while (s >> t && t != "--") fields.push_back(t);
+      }
+      continue;
+    }
+    if (t[0] == '#') continue;
+    std::istringstream r(line);
+    Atom a;
+    a.tid = id;
+    for (const string& f : fields) {
+      double v;
+      if (!(r >> v)) break;
+      if (f == "x") a.x = v;
+      else if (f == "y") a.y = v;
+      else if (f == "z") a.z = v;
+      else if (f == "vx") a.vx = v;
+      else if (f == "vy") a.vy = v;
+      else if (f == "vz") a.vz = v;
+    }
+    atoms.push_back(a);
+  }
+  created = true;
+  return this;
+}
+
+SysBox& AtomContainer::CalcBox() {
+  if (!created) Create();
+  double mn[3] = {1e308, 1e308, 1e308}, mx[3] = {-1e308, -1e308, -1e308};
+  for (const auto& a : atoms) {
+    if (a.flag & FLAG_OUTSIDE) continue;
+    const double p[3] = {a.x, a.y, a.z};
+    for (int d = 0; d < 3; d++) {
+      mn[d] = std::min(mn[d], p[d]);
+      mx[d] = std::max(mx[d], p[d]);
+    }
+  }
+  Box.x0 = mn[0];
+  Box.y0 = mn[1];
+  Box.z0 = mn[2];
+  Box.x1 = mx[0];
+  Box.y1 = mx[1];
+  Box.z1 = mx[2];
+  Box.lx = std::fabs(Box.x1 - Box.x0);
+  Box.ly = std::fabs(Box.y1 - Box.y0);
+  Box.lz = std::fabs(Box.z1 - Box.z0);
+  Box.hlx = Box.lx / 2;
+  Box.hly = Box.ly / 2;
+  Box.hlz = Box.lz / 2;
+  return Box;
+}
+
+Crystalline::Crystalline(const string& orien, int x, int y, int z, const string& mat)
+    : AtomContainer(mat), orientation(orien), xml(x), yml(y), zml(z) {
+  mdassert(param.exist("lattice_constant"),
+           "lattice_constant needed in material definition file: def." + mat);
+  lattice_constant = param.double_value("lattice_constant");
+}
+
+SysBox& Crystalline::CalcBox() {
+  AtomContainer::CalcBox();
+  Box.x0 -= 0.5 * XMLDist;
+  Box.y0 -= 0.5 * YMLDist;
+  Box.z0 -= 0.5 * ZMLDist;
+  Box.x1 += 0.5 * XMLDist;
+  Box.y1 += 0.5 * YMLDist;
+  Box.z1 += 0.5 * ZMLDist;
+  Box.lx = std::fabs(Box.x1 - Box.x0);
+  Box.ly = std::fabs(Box.y1 - Box.y0);
+  Box.lz = std::fabs(Box.z1 - Box.z0);
+  Box.hlx = Box.lx / 2;
+  Box.hly = Box.ly / 2;
+  Box.hlz = Box.lz / 2;
+  return Box;
+}
+
+// fcc "100": unit cells z, y, x; four atoms per cell; z grows downward
+// from 0; monolayers beyond the requested count are dropped.
+AtomContainer* FCC::Create() {
+  if (orientation != "100") die("FCC orientation " + orientation + " is not built");
+  const double h = lattice_constant / 2.;
+  const double rel[4][3] = {{0, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+  XMLDist = YMLDist = ZMLDist = h;
+  const double x0 = 0.0, y0 = 0.0, z1 = 0.0;
+  const double x1 = x0 + h * ((double)xml - 1.);
+  const double y1 = y0 + h * ((double)yml - 1.);
+  const double z0 = z1 - h * ((double)zml - 1.);
+  const int nx = (xml + 1) / 2, ny = (yml + 1) / 2, nz = (zml + 1) / 2;
+  const double dev = 0.0001 * h, off = 2. * h;
+  atoms.clear();
+  for (int i = 0; i < nz; i++) {
+    const double zo = (double)i * off + z0;
+    for (int j = 0; j < ny; j++) {
+      const double yo = (double)j * off + y0;
+      for (int k = 0; k < nx; k++) {
+        const double xo = (double)k * off + x0;
+        for (int l = 0; l < 4; l++) {
+          Atom a;
+          a.tid = id;
+          a.x = xo + rel[l][0] * h;
+          a.y = yo + rel[l][1] * h;
+          a.z = zo + rel[l][2] * h;
+          if (a.x <= x1 + dev && a.y <= y1 + dev && a.z <= z1 + dev) atoms.push_back(a);
+        }
+      }
+    }
+  }
+  created = true;
+  return this;
+}
+
+// diamond "100": xml/4+1 cells per axis, eight atoms per cell
+AtomContainer* Diamond::Create() {
+  if (orientation != "100") die("Diamond orientation " + orientation + " is not built");
+  const int ncx = xml / 4 + 1, ncy = yml / 4 + 1, ncz = zml / 4 + 1;
+  const double a = lattice_constant;
+  XMLDist = YMLDist = ZMLDist = a / 4.0;
+  atoms.clear();
+  atoms.reserve((size_t)ncx * ncy * ncz * 8);
+  double bx = 0.0;
+  for (int i = 0; i < ncx; i++) {
+    double by = 0.0;
+    for (int j = 0; j < ncy; j++) {
+      double bz = 0.0;
+      for (int k = 0; k < ncz; k++) {
+        const double p[8][3] = {{bx, by, bz},
+                                {bx + a / 2., by + a / 2., bz},
+                                {bx + a / 4., by + a / 4., bz - a / 4.},
+                                {bx + 3. * a / 4., by + 3. * a / 4., bz - a / 4.},
+                                {bx, by + a / 2., bz - a / 2.},
+                                {bx + a / 2., by, bz - a / 2.},
+                                {bx + a / 4., by + 3. * a / 4., bz - 3. * a / 4.},
+                                {bx + 3. * a / 4., by + a / 4., bz - 3. * a / 4.}};
+        for (auto& q : p) {
+          Atom at;
+          at.tid = id;
+          at.x = q[0];
+          at.y = q[1];
+          at.z = q[2];
+          atoms.push_back(at);
+        }
+        bz -= a;
+      }
+      by += a;
+    }
+    bx += a;
+  }
+  created = true;
+  return this;
+}
+
+AtomContainer* Projectile::Create() {
+  const double spi = M_PI / 180.0;
+  const double vv = std::sqrt(2.0 * energy_ / M);
+  const double vxy = vv * std::sin(theta_ * spi);
+  Allocate(1);
+  Atom& a = atoms[0];
+  a.flag = FLAG_ACTIVE | FLAG_OUTSIDE;  // never part of the box
+  a.x = tx_;
+  a.y = ty_;
+  a.z = h_;
+  a.vz = -vv * std::cos(theta_ * spi);
+  a.vx = vxy * std::cos(phi_ * spi);
+  a.vy = vxy * std::sin(phi_ * spi);
+  const double pxy = a.z * std::tan(theta_ * spi);
+  a.x -= pxy * std::cos(phi_ * spi);
+  a.y -= pxy * std::sin(phi_ * spi);
+  created = true;
+  return this;
+}
+
+// ------------------------------------------------------------- monitor ---
+void SysMonitor::ReadParameter() {
+  SysParam->peek("monitor.filename", filename);
+  SysParam->peek("monitor.every", every);
+  if (every < 1) every = 1;
+}
+
+void SysMonitor::Init(MDSystem* ws) {
+  Detector::Init(ws);
+  fout.open(filename, std::ios::trunc);
+}
+
+void SysMonitor::Detect() {
+  if (System->Step % every) return;
+  if (!System->Step) return;
+  if (first) {
+    fout << "#@ t: dt: Ek: Ep: E: \n";
+    first = false;
+  }
+  char buf[256];
+  std::snprintf(buf, sizeof buf, "%d %0.6E %0.3E %0.10E %0.10E %0.10E \n", System->Step,
+                System->ElapsedTime, System->Integrator->TimeStep, System->Kinetic,
+                System->Potential - System->BasePotential,
+                System->Kinetic + System->Potential - System->BasePotential);
+  fout << buf;
+  fout.flush();
+}
+
+}  // namespace omd
