@@ -40,8 +40,11 @@ enum { OMDG_TAB_PAIR = 0, OMDG_TAB_EDENS = 1, OMDG_TAB_EMBED = 2 };
 /* potentials */
 enum { OMDG_POT_NONE = 0, OMDG_POT_PAIR = 1, OMDG_POT_EAM = 2,
        OMDG_POT_SW = 3, OMDG_POT_TERSOFF = 4 };
-/* atom flags (base.h:230-232), plus a fixed-atom bit for clamped layers */
-enum { OMDG_FLAG_ACTIVE = 1, OMDG_FLAG_FIXED = 8 };
+/* atom flags: active as FLAG_ACTIVE (base.h:230); FIXED marks the atoms of a
+ * CoordClamp group, DAMP those of a ForceDamper group, LIMIT those of a
+ * RangeLimiter group */
+enum { OMDG_FLAG_ACTIVE = 1, OMDG_FLAG_FIXED = 8, OMDG_FLAG_DAMP = 16,
+       OMDG_FLAG_LIMIT = 32 };
 
 /* Context lifetime.  Replaces the device-less MDSystem/MDIntegrator pair
  * (system.h:139-410, integrator.h:41-69) as the owner of atom state. */
@@ -123,12 +126,44 @@ int omdg_zero_forces(omdg_t ctx);
  * and epot = MeasurePotential; either may be NULL. */
 int omdg_step(omdg_t ctx, int nsteps, double dt, double* ekin, double* epot);
 
+/* Per-step modifies of impact runs, applied by the integrator kernels:
+ * DynamicTimeStep (include/modify/DynamicTimeStep.h:24-72): with
+ * maxpath > 0 and replicas set, every `update` steps each replica takes
+ * dt = min(dt_max, maxpath / (1.1 vmax)) (dt_max if vmax = 0);
+ * ForceDamper (ForceDamper.h:24-52): f -= damp_factor v on OMDG_FLAG_DAMP
+ * atoms; CoordClamp (CoordClamp.h:25-58): f = v = 0 on the axes of
+ * clamp_axes (bit 0/1/2 = x/y/z) for OMDG_FLAG_FIXED atoms; RangeLimiter
+ * (RangeLimiter.h:25-66): range_border >= 0 deactivates OMDG_FLAG_LIMIT atoms
+ * outside the box (of their replica) widened by range_border.  The clamp
+ * mask is explicit: CoordClamp's own parse of "xyz" sets only y and z
+ * (CoordClamp.h:46-48), which a caller mirroring it passes as 6. */
+int omdg_set_modifies(omdg_t ctx, double dt_max, double maxpath, int update,
+                      int clamp_axes, double damp_factor, double range_border);
+
+/* Independent replicas in one context (no counterpart in the reference,
+ * whose force registry is process-global, system.cpp:48, so one process
+ * holds one system).  Atoms [off[r], off[r+1]) of the uploaded set form
+ * replica r, with box lo3/hi3 (3 per replica) and periodic mask pbc[r]. */
+int omdg_set_replicas(omdg_t ctx, int nrep, const int* off, const double* lo3,
+                      const double* hi3, const int* pbc);
+/* nsteps of every replica, each with its own time step: the RunKernel loop
+ * (system.cpp:964-983) with DynamicTimeStep before and RangeLimiter after
+ * the integration.  The first call after omdg_set_replicas starts with
+ * MDSystem::FirstRun (system.cpp:860-873): forces, the force modifies,
+ * RangeLimiter, the maximum speed, then zero forces.  The steps between
+ * two list rebuilds are captured once in a HIP graph and replayed.  After
+ * the run, per replica (arrays of nrep, each may be NULL): elapsed time,
+ * kinetic and potential energy. */
+int omdg_batch_run(omdg_t ctx, int nsteps, double* time, double* ekin,
+                   double* epot);
+
 /* Kernel timing (no counterpart in the reference, whose only timer has
  * one-second resolution, system.cpp:635): with enable = 1 every launch
  * group is bracketed by HIP events on the context's stream.  A call with
  * enable = 0 stops recording and returns, per group, the summed milliseconds
  * and the launch count (groups: 0 list build, 1 density/first pass, 2 force
- * pass, 3 kick+drift, 4 kick, 5 reductions).  max = size of the arrays. */
+ * pass, 3 kick+drift, 4 kick, 5 reductions, 6 replays of captured replica
+ * steps).  max = size of the arrays. */
 int omdg_timing(omdg_t ctx, int enable, int max, double* ms, int* count);
 
 #ifdef __cplusplus

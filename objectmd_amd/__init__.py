@@ -16,7 +16,7 @@ OK, EINVAL, EHIP, ERANGE_HI, ERANGE_LO, EOVERLAP, ENBR, ENODEV = range(8)
 X, V, F, POT, VIR, RHO = range(6)
 TAB_PAIR, TAB_EDENS, TAB_EMBED = range(3)
 POT_PAIR, POT_EAM, POT_SW, POT_TERSOFF = 1, 2, 3, 4
-FLAG_ACTIVE, FLAG_FIXED = 1, 8
+FLAG_ACTIVE, FLAG_FIXED, FLAG_DAMP, FLAG_LIMIT = 1, 8, 16, 32
 
 # every entry point declared in include/omdg.h
 ABI_SYMBOLS = (
@@ -24,9 +24,10 @@ ABI_SYMBOLS = (
     "omdg_upload_atoms", "omdg_set_state", "omdg_download", "omdg_add_table",
     "omdg_load_table", "omdg_set_potential", "omdg_set_sw", "omdg_set_tersoff",
     "omdg_set_neighbor", "omdg_neighbor_build", "omdg_neighbor_export",
-    "omdg_compute", "omdg_zero_forces", "omdg_step", "omdg_timing",
+    "omdg_compute", "omdg_zero_forces", "omdg_step", "omdg_set_modifies",
+    "omdg_set_replicas", "omdg_batch_run", "omdg_timing",
 )
-TIMING_GROUPS = ("build", "pass1", "force", "kick_drift", "kick", "reduce")
+TIMING_GROUPS = ("build", "pass1", "force", "kick_drift", "kick", "reduce", "graph")
 
 _lib = None
 
@@ -68,6 +69,10 @@ def load_library(path=LIB_PATH):
         "omdg_compute": [P, dp, dp],
         "omdg_zero_forces": [P],
         "omdg_step": [P, ctypes.c_int, ctypes.c_double, dp, dp],
+        "omdg_set_modifies": [P, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                              ctypes.c_int, ctypes.c_double, ctypes.c_double],
+        "omdg_set_replicas": [P, ctypes.c_int, ip, dp, dp, ip],
+        "omdg_batch_run": [P, ctypes.c_int, dp, dp, dp],
         "omdg_timing": [P, ctypes.c_int, ctypes.c_int, dp, ip],
     }
     for name, args in sig.items():
@@ -96,6 +101,7 @@ class Engine:
         self.h = ctypes.c_void_p()
         self._check(self.lib.omdg_create(ctypes.byref(self.h), device))
         self.n = 0
+        self.nrep = 0
 
     def _check(self, rc):
         if rc != OK:
@@ -197,6 +203,33 @@ class Engine:
         self._check(self.lib.omdg_step(self.h, int(nsteps), float(dt), ctypes.byref(ek),
                                        ctypes.byref(ep)))
         return ek.value, ep.value
+
+    def set_modifies(self, dt_max=0.0, maxpath=0.0, update=1, clamp_axes=7,
+                     damp_factor=0.0, range_border=-1.0):
+        """DynamicTimeStep / ForceDamper / CoordClamp / RangeLimiter settings."""
+        self._check(self.lib.omdg_set_modifies(
+            self.h, float(dt_max), float(maxpath), int(update), int(clamp_axes),
+            float(damp_factor), float(range_border)))
+
+    def set_replicas(self, off, lo, hi, pbc):
+        """Atoms [off[r], off[r+1]) form replica r with box lo[r]..hi[r]."""
+        off = np.ascontiguousarray(off, dtype=np.int32)
+        nrep = off.size - 1
+        lo = np.ascontiguousarray(lo, dtype=np.float64).reshape(nrep, 3)
+        hi = np.ascontiguousarray(hi, dtype=np.float64).reshape(nrep, 3)
+        pbc = np.ascontiguousarray(np.broadcast_to(pbc, (nrep,)), dtype=np.int32)
+        self._check(self.lib.omdg_set_replicas(self.h, nrep, _iptr(off), _dptr(lo),
+                                               _dptr(hi), _iptr(pbc)))
+        self.nrep = nrep
+
+    def batch_run(self, nsteps):
+        """-> (time, ekin, epot) per replica after nsteps of every replica."""
+        t = np.zeros(self.nrep)
+        ek = np.zeros(self.nrep)
+        ep = np.zeros(self.nrep)
+        self._check(self.lib.omdg_batch_run(self.h, int(nsteps), _dptr(t), _dptr(ek),
+                                            _dptr(ep)))
+        return t, ek, ep
 
     def timing_start(self):
         self._check(self.lib.omdg_timing(self.h, 1, 0, None, None))

@@ -1,0 +1,566 @@
+// Independent replicas in one context (many impact simulations at once).
+//
+// Atoms keep the caller's order: replica r owns atoms [off[r], off[r+1]),
+// each replica has its own box, periodicity, cell grid and time step.  The
+// cells of all replicas form one global range, so one launch of each kernel
+// covers every replica:
+//   bin       wrap periodic coordinates (the reference's image arithmetic),
+//             cell of every atom in its replica's grid, per-cell counts
+//   scan      one workgroup per replica: its cells' starts (a replica's
+//             atoms always fill slots [off[r], off[r+1]) of the cell order)
+//   scatter   atom indices into cell order; cells sorted by atom index
+//   snapshot  FP32 cell-relative positions (type and cell in w)
+//   build     27-cell stencil in the atom's replica grid, list radius per
+//             type pair; FP32 decision outside a thin band, FP64 inside
+// Every kernel of the build skips the replicas whose flag need[r] is clear,
+// so one replica's list is rebuilt without touching the others.  Free
+// dimensions use the grid fixed at set-up: atoms that leave it are clamped
+// into the edge cells, which keeps the stencil complete.
+//
+// One step of the batch (all kernels, no host synchronisation, no memset
+// nodes, so any number of steps is captured into one HIP graph):
+//   k_rep_dt   DynamicTimeStep per replica every `update` steps
+//              (include/modify/DynamicTimeStep.h:61-71)
+//   kick_drift (+ per-replica largest displacement since the replica's
+//              build), k_rep_need (rebuild a replica before one more step
+//              could carry an atom past skin/2), the conditional build,
+//              force pass, kick (+ damper, clamp, range limiter, per-replica
+//              max speed), k_rep_tick (elapsed time per replica, step count)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "omdg_internal.h"
+
+namespace omdg {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ void k_fill(int* __restrict__ p, long n, int v) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// zero the counts of the cells of rebuilt replicas
+__global__ void k_clear_cells(int* __restrict__ count, const int* __restrict__ cell_rep,
+                              const int* __restrict__ need, long nc) {
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < nc;
+       c += (long)gridDim.x * blockDim.x)
+    if (need[cell_rep[c]]) count[c] = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_batch_bin(double* __restrict__ x, int n,
+                                                      const int* __restrict__ rep,
+                                                      const int* __restrict__ need,
+                                                      const RepGrid* __restrict__ grid,
+                                                      int* __restrict__ cellid,
+                                                      int* __restrict__ count,
+                                                      int* __restrict__ stat) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !need[rep[i]]) return;
+  const RepGrid& g = grid[rep[i]];
+  int k[3];
+  for (int d = 0; d < 3; d++) {
+    double v = x[(size_t)d * n + i];
+    if ((g.pbc >> d) & 1) {
+      // the image inside the box becomes the atom (systemgrid.cpp:479-506)
+      int guard = 0;
+      while (v < g.lo[d] && guard < 2) { v = v + g.L[d]; guard++; }
+      while (v >= g.lo[d] + g.L[d] && guard < 2) { v = v + (-g.L[d]); guard++; }
+      if (v < g.lo[d] || v >= g.lo[d] + g.L[d]) atomicOr(stat + ST_ERR, ERR_WRAP);
+      x[(size_t)d * n + i] = v;
+    }
+    int kk = (int)floor((v - g.origin[d]) * g.inv_h[d]);
+    k[d] = kk < 0 ? 0 : (kk > g.M[d] - 1 ? g.M[d] - 1 : kk);
+  }
+  const int c = (int)g.cell_off + k[0] + g.M[0] * (k[1] + g.M[1] * k[2]);
+  cellid[i] = c;
+  atomicAdd(count + c, 1);
+}
+
+// cell starts of one replica per workgroup; counts reset for the scatter
+__global__ void __launch_bounds__(kBlock) k_rep_scan(const RepGrid* __restrict__ grid,
+                                                     const int* __restrict__ off,
+                                                     const int* __restrict__ need, int nrep,
+                                                     int* __restrict__ count,
+                                                     int* __restrict__ start) {
+  const int r = blockIdx.x;
+  if (r >= nrep || !need[r]) return;  // uniform per block
+  const RepGrid& g = grid[r];
+  const int c0 = (int)g.cell_off, nc = g.M[0] * g.M[1] * g.M[2];
+  const int per = (nc + kBlock - 1) / kBlock;
+  const int b = c0 + (int)threadIdx.x * per, e = min(c0 + nc, b + per);
+  int s = 0;
+  for (int c = b; c < e; c++) s += count[c];
+  __shared__ int sh[kBlock];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < kBlock; o <<= 1) {
+    const int v = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = off[r] + sh[threadIdx.x] - s;
+  for (int c = b; c < e; c++) {
+    const int v = count[c];
+    start[c] = run;
+    run += v;
+    count[c] = 0;
+  }
+  if (r == nrep - 1 && threadIdx.x == 0) start[c0 + nc] = off[nrep];
+}
+
+__global__ void k_batch_scatter(const int* __restrict__ cellid, int n,
+                                const int* __restrict__ rep, const int* __restrict__ need,
+                                const int* __restrict__ start, int* __restrict__ fill,
+                                int* __restrict__ perm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !need[rep[i]]) return;
+  const int c = cellid[i];
+  perm[start[c] + atomicAdd(fill + c, 1)] = i;
+}
+
+// each cell in ascending atom index, one wavefront per cell (rank sort)
+__global__ void __launch_bounds__(256) k_batch_cell_sort(int* __restrict__ perm,
+                                                         const int* __restrict__ start, int nc,
+                                                         const int* __restrict__ cell_rep,
+                                                         const int* __restrict__ need) {
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x % 64;
+  if (c >= nc || !need[cell_rep[c]]) return;  // uniform per wavefront
+  const int b = start[c], e = start[c + 1], m = e - b;
+  if (m <= 64) {
+    const int p = lane < m ? perm[b + lane] : 0x7fffffff;
+    int rank = 0;
+    for (int u = 0; u < m; u++) rank += (__shfl(p, u, 64) < p) ? 1 : 0;
+    if (lane < m) perm[b + rank] = p;
+  } else if (lane == 0) {
+    for (int i = b + 1; i < e; i++) {
+      const int p = perm[i];
+      int j = i - 1;
+      while (j >= b && perm[j] > p) {
+        perm[j + 1] = perm[j];
+        j--;
+      }
+      perm[j + 1] = p;
+    }
+  }
+}
+
+struct BBArgs {
+  const double* x;
+  const int* type;
+  const int* flag;
+  const int* rep;
+  const int* need;
+  const RepGrid* grid;
+  const int* cellid;
+  const int* start;
+  const int* perm;
+  uint32_t* nbr;
+  int* nnbr;
+  float4* q;
+  float4* qb;
+  int* stat;
+  int n;
+  int maxnbr;
+  int has_flags;
+  double rl2[kMaxTypes * kMaxTypes];
+  float rl2_lo[kMaxTypes * kMaxTypes], rl2_hi[kMaxTypes * kMaxTypes];
+};
+
+__device__ inline void local_cell(const RepGrid& g, int c, int& k0, int& k1, int& k2) {
+  const int l = c - (int)g.cell_off;
+  k0 = l % g.M[0];
+  k1 = (l / g.M[0]) % g.M[1];
+  k2 = l / (g.M[0] * g.M[1]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_batch_snapshot(BBArgs a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n || !a.need[a.rep[s]]) return;
+  const int n = a.n;
+  const RepGrid& g = a.grid[a.rep[s]];
+  int k0, k1, k2;
+  local_cell(g, a.cellid[s], k0, k1, k2);
+  float4 qv;
+  qv.x = (float)(a.x[s] - (g.origin[0] + (double)k0 * g.h[0]));
+  qv.y = (float)(a.x[(size_t)n + s] - (g.origin[1] + (double)k1 * g.h[1]));
+  qv.z = (float)(a.x[2 * (size_t)n + s] - (g.origin[2] + (double)k2 * g.h[2]));
+  qv.w = snap_word(a.type[s], k0, k1, k2);
+  a.q[s] = qv;
+  a.qb[s] = qv;
+}
+
+__device__ inline double shifted(double x, int s, double L) {
+  return s == 0 ? x : (s < 0 ? x + (-L) : x + L);
+}
+
+__global__ void __launch_bounds__(kBlock) k_batch_build(BBArgs a) {
+#pragma clang fp contract(off)
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n || !a.need[a.rep[s]]) return;
+  const int n = a.n;
+  if (a.has_flags && !(a.flag[s] & OMDG_FLAG_ACTIVE)) {
+    a.nnbr[s] = 0;
+    return;
+  }
+  const RepGrid& g = a.grid[a.rep[s]];
+  const float4 qi = a.q[s];
+  const int ti = snap_type(qi.w);
+  int k0, k1, k2;
+  snap_cell(qi.w, k0, k1, k2);
+  const double xi = a.x[s], yi = a.x[(size_t)n + s], zi = a.x[2 * (size_t)n + s];
+  const float hx = (float)g.h[0], hy = (float)g.h[1], hz = (float)g.h[2];
+  int cnt = 0;
+  for (int dz = -1; dz <= 1; dz++) {
+    int kz = k2 + dz, sz = 0;
+    if ((g.pbc >> 2) & 1) {
+      if (kz < 0) { kz += g.M[2]; sz = -1; }
+      else if (kz >= g.M[2]) { kz -= g.M[2]; sz = 1; }
+    } else if (kz < 0 || kz >= g.M[2]) {
+      continue;
+    }
+    for (int dy = -1; dy <= 1; dy++) {
+      int ky = k1 + dy, sy = 0;
+      if ((g.pbc >> 1) & 1) {
+        if (ky < 0) { ky += g.M[1]; sy = -1; }
+        else if (ky >= g.M[1]) { ky -= g.M[1]; sy = 1; }
+      } else if (ky < 0 || ky >= g.M[1]) {
+        continue;
+      }
+      for (int dx = -1; dx <= 1; dx++) {
+        int kx = k0 + dx, sx = 0;
+        if (g.pbc & 1) {
+          if (kx < 0) { kx += g.M[0]; sx = -1; }
+          else if (kx >= g.M[0]) { kx -= g.M[0]; sx = 1; }
+        } else if (kx < 0 || kx >= g.M[0]) {
+          continue;
+        }
+        const int cc = (int)g.cell_off + kx + g.M[0] * (ky + g.M[1] * kz);
+        const uint32_t img = (uint32_t)((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1));
+        const bool noshift = (sx | sy | sz) == 0;
+        const float ox = (float)dx * hx, oy = (float)dy * hy, oz = (float)dz * hz;
+        const int e = a.start[cc + 1];
+        for (int u = a.start[cc]; u < e; u++) {
+          const int t = a.perm[u];
+          if (t == s && noshift) continue;
+          if (a.has_flags && !(a.flag[t] & OMDG_FLAG_ACTIVE)) continue;
+          const float4 qt = a.q[t];
+          const int pt = ti * kMaxTypes + snap_type(qt.w);
+          const float fx = qi.x - qt.x - ox, fy = qi.y - qt.y - oy, fz = qi.z - qt.z - oz;
+          const float r2f = fx * fx + fy * fy + fz * fz;
+          bool in;
+          if (r2f < a.rl2_lo[pt]) {
+            in = true;
+          } else if (r2f > a.rl2_hi[pt]) {
+            in = false;
+          } else {
+            const double ddx = xi - shifted(a.x[t], sx, g.L[0]);
+            const double ddy = yi - shifted(a.x[(size_t)n + t], sy, g.L[1]);
+            const double ddz = zi - shifted(a.x[2 * (size_t)n + t], sz, g.L[2]);
+            in = ddx * ddx + ddy * ddy + ddz * ddz <= a.rl2[pt];
+          }
+          if (in) {
+            if (cnt < a.maxnbr) a.nbr[nidx(s, cnt, n)] = (uint32_t)t | (img << kImgShift);
+            cnt++;
+          }
+        }
+      }
+    }
+  }
+  a.nnbr[s] = cnt < a.maxnbr ? cnt : a.maxnbr;
+  if (cnt > a.maxnbr) atomicOr(a.stat + ST_ERR, ERR_NBR_OVERFLOW);
+  atomicMax(a.stat + ST_MAXNBR, cnt);
+}
+
+// ---- per-replica kernels ----------------------------------------------------
+struct RepArgs {
+  const double* v;
+  const int* type;
+  const int* flag;
+  const float* pot;
+  const int* off;
+  double* dt;
+  double* time;
+  double* e;
+  unsigned long long* v2;  // max |v|^2 of the last kick (double bits)
+  double* vcur;            // the same, kept for the rebuild decision
+  unsigned int* d2;        // max displacement^2 since the build (float bits)
+  int* need;
+  int* step;
+  int* stat;
+  int n, nrep;
+  int has_flags;
+  double mass[kMaxTypes];
+  double dtmax, maxpath;
+  int update;
+  double half_skin;
+};
+
+__device__ inline double block_reduce(double v, bool is_max) {
+  __shared__ double sh[kBlock / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    const double w = __shfl_xor(v, o, 64);
+    v = is_max ? fmax(v, w) : v + w;
+  }
+  const int wv = threadIdx.x / 64, l = threadIdx.x % 64;
+  __syncthreads();
+  if (l == 0) sh[wv] = v;
+  __syncthreads();
+  double r = sh[0];
+  for (int k = 1; k < kBlock / 64; k++) r = is_max ? fmax(r, sh[k]) : r + sh[k];
+  return r;
+}
+
+// SqrMaxVelocity of MDSystem::MeasureKinetic (system.cpp:1077-1094) per
+// replica from the current velocities of the active atoms
+__global__ void __launch_bounds__(kBlock) k_rep_vinit(RepArgs a) {
+  const int r = blockIdx.x;
+  if (r >= a.nrep) return;  // uniform per block
+  const int n = a.n;
+  double vm = 0.0;
+  for (int s = a.off[r] + threadIdx.x; s < a.off[r + 1]; s += blockDim.x) {
+    if (a.has_flags && !(a.flag[s] & OMDG_FLAG_ACTIVE)) continue;
+    const double vx = a.v[s], vy = a.v[(size_t)n + s], vz = a.v[2 * (size_t)n + s];
+    vm = fmax(vm, vx * vx + vy * vy + vz * vz);
+  }
+  vm = block_reduce(vm, true);
+  if (threadIdx.x == 0) a.v2[r] = (unsigned long long)__double_as_longlong(vm);
+}
+
+// DynamicTimeStep::PreIntegration (include/modify/DynamicTimeStep.h:61-71):
+// every `update` steps dt = min(dt_max, maxpath / (1.1 vmax)), dt_max when
+// vmax = 0, with vmax of the previous step's kick.  The maximum is cleared
+// every step for the next kick to fill.
+__global__ void k_rep_dt(RepArgs a) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.nrep) return;
+  const double vm = __longlong_as_double((long long)a.v2[r]);
+  a.v2[r] = 0ull;
+  a.vcur[r] = vm;
+  if (!(a.maxpath > 0.0) || (*a.step) % a.update != 0) return;
+  double nd = a.dtmax;
+  if (vm > 0.0) {
+    nd = a.maxpath / (sqrt(vm) * 1.10);
+    if (nd > a.dtmax) nd = a.dtmax;
+  }
+  a.dt[r] = nd;
+}
+
+// Rebuild replica r now if one more step could carry an atom past skin/2:
+// largest displacement since its build plus 1.5x the largest speed times
+// its time step (at least maxpath).  The displacement maximum is cleared for
+// the next drift to fill.
+__global__ void k_rep_need(RepArgs a) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.nrep) return;
+  const double d = sqrt((double)__uint_as_float(a.d2[r]));
+  a.d2[r] = 0u;
+  double ahead = 1.5 * sqrt(a.vcur[r]) * a.dt[r];
+  if (a.maxpath > ahead) ahead = a.maxpath;
+  const int nd = d + ahead >= a.half_skin ? 1 : 0;
+  a.need[r] = nd;
+  if (nd) atomicAdd(a.stat + ST_REBUILDS, 1);
+}
+
+__global__ void k_rep_tick(RepArgs a) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < a.nrep) a.time[r] += a.dt[r];
+  if (r == 0) *a.step += 1;
+}
+
+__global__ void __launch_bounds__(kBlock) k_rep_energy(RepArgs a) {
+  const int r = blockIdx.x;
+  if (r >= a.nrep) return;
+  const int n = a.n;
+  double ek = 0.0, ep = 0.0;
+  for (int s = a.off[r] + threadIdx.x; s < a.off[r + 1]; s += blockDim.x) {
+    if (a.has_flags && !(a.flag[s] & OMDG_FLAG_ACTIVE)) continue;
+    const double vx = a.v[s], vy = a.v[(size_t)n + s], vz = a.v[2 * (size_t)n + s];
+    ek += a.mass[a.type[s]] * (vx * vx + vy * vy + vz * vz);
+    ep += (double)a.pot[s];
+  }
+  ek = block_reduce(ek, false);
+  ep = block_reduce(ep, false);
+  if (threadIdx.x == 0) {
+    a.e[2 * r] = 0.5 * ek;
+    a.e[2 * r + 1] = ep;
+  }
+}
+
+RepArgs rep_args(Ctx& c) {
+  RepArgs a{};
+  a.v = c.v[c.cur];
+  a.type = c.type[c.cur];
+  a.flag = c.flag[c.cur];
+  a.pot = c.pot32;
+  a.off = c.rep_offd;
+  a.dt = c.rep_dt;
+  a.time = c.rep_time;
+  a.e = c.rep_e;
+  a.v2 = c.rep_v2;
+  a.vcur = c.rep_vcur;
+  a.d2 = c.rep_d2;
+  a.need = c.rep_need;
+  a.step = c.dstep;
+  a.stat = c.dstat;
+  a.half_skin = 0.5 * c.skin;
+  a.n = c.n;
+  a.nrep = c.nrep;
+  a.has_flags = c.has_flags;
+  for (int t = 0; t < kMaxTypes; t++) a.mass[t] = c.mass[t];
+  a.dtmax = c.dyn_dtmax;
+  a.maxpath = c.dyn_maxpath;
+  a.update = c.dyn_update > 0 ? c.dyn_update : 1;
+  return a;
+}
+
+}  // namespace
+
+// Per-replica grids: periodic dimensions tile the replica box with cells no
+// smaller than the list radius; free dimensions cover the atoms at set-up.
+void batch_setup_grids(Ctx& c, const double* lo3, const double* hi3, const int* pbc) {
+  std::vector<double> x(3 * (size_t)c.n);
+  OMDG_HIP(hipMemcpy(x.data(), c.x[c.cur], sizeof(double) * 3 * c.n, hipMemcpyDeviceToHost));
+  const double rl = c.rlist;
+  c.rep_grid_h.assign(c.nrep, RepGrid{});
+  long cells = 0;
+  for (int r = 0; r < c.nrep; r++) {
+    RepGrid& g = c.rep_grid_h[r];
+    g.pbc = pbc[r] & 7;
+    for (int d = 0; d < 3; d++) {
+      g.lo[d] = lo3[3 * r + d];
+      g.hi[d] = hi3[3 * r + d];
+      g.L[d] = std::fabs(g.hi[d] - g.lo[d]);
+      double org, ext;
+      if ((g.pbc >> d) & 1) {
+        org = g.lo[d];
+        ext = g.L[d];
+        if (ext < rl) throw ApiFail(OMDG_EINVAL, "replica box shorter than the list radius");
+      } else {
+        double mn = 1e308, mx = -1e308;
+        for (int i = c.rep_off[r]; i < c.rep_off[r + 1]; i++) {
+          mn = std::min(mn, x[(size_t)d * c.n + i]);
+          mx = std::max(mx, x[(size_t)d * c.n + i]);
+        }
+        org = mn;
+        ext = mx - mn;
+      }
+      int M = (int)(ext / rl);
+      M = std::max(1, std::min(M, 1023));
+      g.M[d] = M;
+      g.origin[d] = org;
+      g.h[d] = ext > 0 ? ext / M : 1.0;
+      g.inv_h[d] = 1.0 / g.h[d];
+    }
+    g.cell_off = cells;
+    cells += (long)g.M[0] * g.M[1] * g.M[2];
+  }
+  if (cells >= (1L << 31)) throw ApiFail(OMDG_EINVAL, "too many cells");
+  c.rep_ncell = cells;
+  std::vector<float4> hh(c.nrep);
+  for (int r = 0; r < c.nrep; r++)
+    hh[r] = make_float4((float)c.rep_grid_h[r].h[0], (float)c.rep_grid_h[r].h[1],
+                        (float)c.rep_grid_h[r].h[2], 0.f);
+  auto realloc = [](auto*& p, size_t bytes) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&p), bytes));
+  };
+  realloc(c.rep_grid, sizeof(RepGrid) * c.nrep);
+  realloc(c.rep_h, sizeof(float4) * c.nrep);
+  OMDG_HIP(hipMemcpy(c.rep_grid, c.rep_grid_h.data(), sizeof(RepGrid) * c.nrep,
+                     hipMemcpyHostToDevice));
+  OMDG_HIP(hipMemcpy(c.rep_h, hh.data(), sizeof(float4) * c.nrep, hipMemcpyHostToDevice));
+  if (cells + 1 > c.cap_cells) {
+    realloc(c.cell_start, sizeof(int) * (cells + 1));
+    realloc(c.cell_fill, sizeof(int) * (cells + 1));
+    c.cap_cells = cells + 1;
+  }
+  // replica of every cell, for the kernels that run over cells
+  std::vector<int> cr(cells);
+  for (int r = 0; r < c.nrep; r++) {
+    const RepGrid& g = c.rep_grid_h[r];
+    const long m = (long)g.M[0] * g.M[1] * g.M[2];
+    std::fill(cr.begin() + g.cell_off, cr.begin() + g.cell_off + m, r);
+  }
+  realloc(c.cell_rep, sizeof(int) * std::max(1L, cells));
+  OMDG_HIP(hipMemcpy(c.cell_rep, cr.data(), sizeof(int) * cells, hipMemcpyHostToDevice));
+  // counts start at zero; each build clears the cells it rebuilds again
+  k_fill<<<std::min(1024, grid_blocks(cells + 1, kBlock)), kBlock, 0, c.stream>>>(c.cell_fill,
+                                                                                  cells + 1, 0);
+  OMDG_HIP(hipGetLastError());
+}
+
+void batch_build(Ctx& c, bool all) {
+  const int n = c.n;
+  const int nc = (int)c.rep_ncell;
+  if (all) k_fill<<<grid_blocks(c.nrep, kBlock), kBlock, 0, c.stream>>>(c.rep_need, c.nrep, 1);
+  k_clear_cells<<<std::min(2048, grid_blocks(nc, kBlock)), kBlock, 0, c.stream>>>(
+      c.cell_fill, c.cell_rep, c.rep_need, nc);
+  k_batch_bin<<<grid_blocks(n, kBlock), kBlock, 0, c.stream>>>(
+      c.x[c.cur], n, c.rep, c.rep_need, c.rep_grid, c.cellid, c.cell_fill, c.dstat);
+  k_rep_scan<<<c.nrep, kBlock, 0, c.stream>>>(c.rep_grid, c.rep_offd, c.rep_need, c.nrep,
+                                              c.cell_fill, c.cell_start);
+  k_batch_scatter<<<grid_blocks(n, kBlock), kBlock, 0, c.stream>>>(
+      c.cellid, n, c.rep, c.rep_need, c.cell_start, c.cell_fill, c.perm);
+  k_batch_cell_sort<<<grid_blocks((long)nc * 64, 256), 256, 0, c.stream>>>(
+      c.perm, c.cell_start, nc, c.cell_rep, c.rep_need);
+  BBArgs a{};
+  a.x = c.x[c.cur];
+  a.type = c.type[c.cur];
+  a.flag = c.flag[c.cur];
+  a.rep = c.rep;
+  a.need = c.rep_need;
+  a.grid = c.rep_grid;
+  a.cellid = c.cellid;
+  a.start = c.cell_start;
+  a.perm = c.perm;
+  a.nbr = c.nbr;
+  a.nnbr = c.nnbr;
+  a.q = c.q;
+  a.qb = c.qb;
+  a.stat = c.dstat;
+  a.n = n;
+  a.maxnbr = c.maxnbr;
+  a.has_flags = c.has_flags ? 1 : 0;
+  for (int p = 0; p < kMaxTypes * kMaxTypes; p++) {
+    a.rl2[p] = (double)c.rl2_type[p];
+    a.rl2_lo[p] = (float)(a.rl2[p] * (1.0 - 3e-5));
+    a.rl2_hi[p] = (float)(a.rl2[p] * (1.0 + 3e-5));
+  }
+  k_batch_snapshot<<<grid_blocks(n, kBlock), kBlock, 0, c.stream>>>(a);
+  k_batch_build<<<grid_blocks(n, kBlock), kBlock, 0, c.stream>>>(a);
+  OMDG_HIP(hipGetLastError());
+}
+
+void batch_rep_dt(Ctx& c) {
+  k_rep_dt<<<grid_blocks(c.nrep, kBlock), kBlock, 0, c.stream>>>(rep_args(c));
+}
+
+void batch_rep_need(Ctx& c) {
+  k_rep_need<<<grid_blocks(c.nrep, kBlock), kBlock, 0, c.stream>>>(rep_args(c));
+}
+
+void batch_rep_vinit(Ctx& c) {
+  k_rep_vinit<<<c.nrep, kBlock, 0, c.stream>>>(rep_args(c));
+  OMDG_HIP(hipGetLastError());
+}
+
+void batch_rep_tick(Ctx& c) {
+  k_rep_tick<<<grid_blocks(c.nrep, kBlock), kBlock, 0, c.stream>>>(rep_args(c));
+}
+
+void batch_rep_energy(Ctx& c) {
+  k_rep_energy<<<c.nrep, kBlock, 0, c.stream>>>(rep_args(c));
+  OMDG_HIP(hipGetLastError());
+}
+
+}  // namespace omdg

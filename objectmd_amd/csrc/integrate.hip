@@ -1,12 +1,19 @@
 // Velocity Verlet in FP64 (MDIntegrator::Integrate,
-// src/omd/integrator.cpp:225-259) and the global reductions of
-// MDSystem::MeasurePotential/MeasureKinetic (src/omd/system.cpp:1060-1094).
+// src/omd/integrator.cpp:225-259), the per-step modifies of impact runs and
+// the global reductions of MDSystem::MeasurePotential/MeasureKinetic
+// (src/omd/system.cpp:1060-1094).
 //
 // kick_drift: v += (dt/2 f)/m; x += dt v; refresh the FP32 cell-relative
-//             snapshot; track the largest displacement since the last list
-//             build and the largest speed (list-validity check on the host).
-// kick:       v += (dt/2 f)/m.
-// Inactive atoms are not moved; fixed atoms (flag bit 8) keep x and v.
+//             snapshot; per-block maxima of the displacement since the last
+//             list build and of the speed (list check on the host).
+// kick:       the POST_FORCE modifies, ForceDamper f -= gamma v on damped
+//             atoms (include/modify/ForceDamper.h:39-46) and CoordClamp
+//             f = v = 0 on the clamped axes of fixed atoms (CoordClamp.h:50-56;
+//             both orders give the same result); then v += (dt/2 f)/m; then
+//             RangeLimiter deactivates limited atoms outside the box widened
+//             by the border (RangeLimiter.h:54-65).
+// Inactive atoms are not moved.  With replicas every atom takes the time
+// step and cell grid of its replica.
 #include <hip/hip_runtime.h>
 
 #include "omdg_internal.h"
@@ -20,11 +27,10 @@ constexpr int kBlock = 256;
 struct IntArgs {
   double* x;
   double* v;
-  const float* f32;
-  const double* f64;
+  float* f32;
+  double* f64;
   const int* type;
-  const int* flag;
-  const int* cellid;
+  int* flag;
   float4* q;
   const float4* qb;
   const double* xb;
@@ -34,9 +40,20 @@ struct IntArgs {
   double dt, hdt;
   double mass[kMaxTypes];
   double hdt_over_m[kMaxTypes];  // FP32-force mode: (dt/2)/m
-  int M[3];
   double origin[3], h[3];
   float half_skin2;
+  // modifies
+  int clamp_axes;
+  double damp;
+  int range_on;
+  double range_lo[3], range_hi[3];
+  // replicas
+  const int* rep;
+  const double* rep_dt;
+  const RepGrid* rep_grid;
+  unsigned long long* rep_v2;  // per-replica max |v|^2 (double bits)
+  unsigned int* rep_d2;        // per-replica max displacement^2 (float bits)
+  int first;                   // FirstRun: modifies only, no velocity change
 };
 
 __device__ inline float wave_max(float v) {
@@ -84,12 +101,10 @@ __global__ void __launch_bounds__(kBlock) k_status_max(const float* part, int nb
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int k = 1; k < kBlock / 64; k++) {
+    for (int k = 0; k < kBlock / 64; k++) {
       a = fmaxf(a, sa[k]);
       b = fmaxf(b, sb[k]);
     }
-    a = fmaxf(a, sa[0]);
-    b = fmaxf(b, sb[0]);
     // non-negative floats order like their bit patterns
     stat[ST_DISP2] = __float_as_int(a);
     stat[ST_VMAX2] = __float_as_int(b);
@@ -98,18 +113,17 @@ __global__ void __launch_bounds__(kBlock) k_status_max(const float* part, int nb
 
 // FP64 updates are rounded as written (no fused multiply-add), like the
 // reference's x86-64 build.
-template <bool FP64F, bool FLAGS>
+template <bool FP64F, bool FLAGS, bool BATCH>
 __global__ void __launch_bounds__(kBlock) k_kick_drift(IntArgs a) {
 #pragma clang fp contract(off)
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   float disp2 = 0.f, v2max = 0.f;
   if (s < a.n) {
     const int n = a.n;
-    bool move = true;
-    if (FLAGS) {
-      const int fl = a.flag[s];
-      move = (fl & OMDG_FLAG_ACTIVE) && !(fl & OMDG_FLAG_FIXED);
-    }
+    const bool move = FLAGS ? (a.flag[s] & OMDG_FLAG_ACTIVE) != 0 : true;
+    const int r = BATCH ? a.rep[s] : 0;
+    const double dt = BATCH ? a.rep_dt[r] : a.dt;
+    const double hdt = 0.5 * dt;
     float4 qb;
     int ty;
     if (FP64F) {
@@ -119,7 +133,7 @@ __global__ void __launch_bounds__(kBlock) k_kick_drift(IntArgs a) {
       ty = snap_type(qb.w);
     }
     const double m = a.mass[ty];
-    const double hdt_m = a.hdt_over_m[ty];
+    const double hdt_m = BATCH ? hdt / m : a.hdt_over_m[ty];
     double xs[3];
     for (int d = 0; d < 3; d++) {
       const size_t k = (size_t)d * n + s;
@@ -127,11 +141,11 @@ __global__ void __launch_bounds__(kBlock) k_kick_drift(IntArgs a) {
       if (move) {
         double v;
         if (FP64F) {
-          v = a.v[k] + a.hdt * a.f64[k] / m;  // integrator.cpp:236 order
+          v = a.v[k] + hdt * a.f64[k] / m;  // integrator.cpp:236 order
         } else {
           v = a.v[k] + (double)a.f32[k] * hdt_m;
         }
-        x = x + a.dt * v;
+        x = x + dt * v;
         a.v[k] = v;
         a.x[k] = x;
         v2max += (float)(v * v);
@@ -148,47 +162,135 @@ __global__ void __launch_bounds__(kBlock) k_kick_drift(IntArgs a) {
     } else {
       int k0, k1, k2;
       snap_cell(qb.w, k0, k1, k2);
+      const double* org = BATCH ? a.rep_grid[r].origin : a.origin;
+      const double* hh = BATCH ? a.rep_grid[r].h : a.h;
       float4 qn;
-      qn.x = (float)(xs[0] - (a.origin[0] + (double)k0 * a.h[0]));
-      qn.y = (float)(xs[1] - (a.origin[1] + (double)k1 * a.h[1]));
-      qn.z = (float)(xs[2] - (a.origin[2] + (double)k2 * a.h[2]));
+      qn.x = (float)(xs[0] - (org[0] + (double)k0 * hh[0]));
+      qn.y = (float)(xs[1] - (org[1] + (double)k1 * hh[1]));
+      qn.z = (float)(xs[2] - (org[2] + (double)k2 * hh[2]));
       qn.w = qb.w;
       a.q[s] = qn;
       const float ex = qn.x - qb.x, ey = qn.y - qb.y, ez = qn.z - qb.z;
       disp2 = ex * ex + ey * ey + ez * ez;
     }
-    if (disp2 > a.half_skin2) atomicOr(a.stat + ST_ERR, ERR_SKIN);
+    if (move && disp2 > a.half_skin2) atomicOr(a.stat + ST_ERR, ERR_SKIN);
+    if (!move) disp2 = 0.f;
   }
-  block_max_store(disp2, v2max, a.part_max);
+  if (BATCH) {
+    // per-replica maximum for the device-side rebuild decision
+    const int r = s < a.n ? a.rep[s] : -1;
+    const int r0 = __shfl(r, 0, 64);
+    if (__all(r == r0)) {
+      const float m = wave_max(disp2);
+      if ((threadIdx.x & 63) == 0 && r0 >= 0 && m > 0.f) atomicMax(a.rep_d2 + r0, __float_as_uint(m));
+    } else if (r >= 0 && disp2 > 0.f) {
+      atomicMax(a.rep_d2 + r, __float_as_uint(disp2));
+    }
+  } else {
+    block_max_store(disp2, v2max, a.part_max);
+  }
 }
 
-template <bool FP64F, bool FLAGS>
-__global__ void __launch_bounds__(kBlock) k_kick(IntArgs a) {
+// One atom of the kick.  The force modifies act on every atom of their
+// groups, active or not (ForceDamper.h:39-46, CoordClamp.h:50-56); the
+// velocity update and the range check only on active atoms.  Returns |v|^2
+// of an active atom after the update (0 otherwise): MeasureKinetic's
+// SqrMaxVelocity (system.cpp:1077-1094) runs before RangeLimiter
+// (system.cpp:970-973), so atoms removed in this step still count.
+template <bool FP64F, bool FLAGS, bool BATCH>
+__device__ inline double kick_one(const IntArgs& a, int s, int r) {
 #pragma clang fp contract(off)
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.n) return;
-  if (FLAGS) {
-    const int fl = a.flag[s];
-    if (!(fl & OMDG_FLAG_ACTIVE) || (fl & OMDG_FLAG_FIXED)) return;
-  }
   const int n = a.n;
+  const int fl = FLAGS ? a.flag[s] : OMDG_FLAG_ACTIVE;
+  const bool active = (fl & OMDG_FLAG_ACTIVE) != 0;
   const int ty = a.type[s];
+  const double dt = a.first ? 0.0 : (BATCH ? a.rep_dt[r] : a.dt);
+  const double hdt = 0.5 * dt;
+  double v2 = 0.0;
   for (int d = 0; d < 3; d++) {
     const size_t k = (size_t)d * n + s;
-    if (FP64F) a.v[k] = a.v[k] + a.hdt * a.f64[k] / a.mass[ty];  // integrator.cpp:253
-    else a.v[k] = a.v[k] + (double)a.f32[k] * a.hdt_over_m[ty];
+    double f = FP64F ? a.f64[k] : (double)a.f32[k];
+    double v = a.v[k];
+    if (FLAGS) {
+      bool changed = false;
+      if ((fl & OMDG_FLAG_DAMP) && a.damp != 0.0) {
+        f = f - a.damp * v;
+        changed = true;
+      }
+      if ((fl & OMDG_FLAG_FIXED) && ((a.clamp_axes >> d) & 1)) {
+        f = 0.0;
+        v = 0.0;
+        changed = true;
+        if (!active) a.v[k] = 0.0;
+      }
+      if (changed) {
+        if (FP64F) a.f64[k] = f;
+        else a.f32[k] = (float)f;
+      }
+    }
+    if (!active) continue;
+    if (FP64F) v = v + hdt * f / a.mass[ty];  // integrator.cpp:253
+    else v = v + f * (BATCH ? hdt / a.mass[ty] : a.hdt_over_m[ty]);
+    a.v[k] = v;
+    v2 += v * v;
   }
+  if (FLAGS && active && a.range_on && (fl & OMDG_FLAG_LIMIT)) {
+    const double* lo = a.range_lo;
+    const double* hi = a.range_hi;
+    double rlo[3], rhi[3];
+    if (BATCH) {
+      const RepGrid& g = a.rep_grid[r];
+      for (int d = 0; d < 3; d++) {
+        rlo[d] = g.lo[d] - a.range_lo[0];
+        rhi[d] = g.hi[d] + a.range_lo[0];
+      }
+      lo = rlo;
+      hi = rhi;
+    }
+    for (int d = 0; d < 3; d++) {
+      const double x = a.x[(size_t)d * n + s];
+      if (x < lo[d] || x > hi[d]) {
+        a.flag[s] = fl & ~OMDG_FLAG_ACTIVE;
+        break;
+      }
+    }
+  }
+  return v2;
+}
+
+// Per-replica maximum of |v|^2: one atomic per wavefront when the wavefront
+// lies inside one replica (positive doubles order like their bit patterns).
+__device__ inline void rep_max_store(unsigned long long* out, int r, double v2) {
+  const int r0 = __shfl(r, 0, 64);
+  if (__all(r == r0)) {
+    for (int o = 32; o > 0; o >>= 1) v2 = fmax(v2, __shfl_xor(v2, o, 64));
+    if ((threadIdx.x & 63) == 0 && r0 >= 0 && v2 > 0.0)
+      atomicMax(out + r0, (unsigned long long)__double_as_longlong(v2));
+  } else if (r >= 0 && v2 > 0.0) {
+    atomicMax(out + r, (unsigned long long)__double_as_longlong(v2));
+  }
+}
+
+template <bool FP64F, bool FLAGS, bool BATCH>
+__global__ void __launch_bounds__(kBlock) k_kick(IntArgs a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int r = -1;
+  double v2 = 0.0;
+  if (s < a.n) {
+    r = BATCH ? a.rep[s] : 0;
+    v2 = kick_one<FP64F, FLAGS, BATCH>(a, s, r);
+  }
+  if (BATCH) rep_max_store(a.rep_v2, r, v2);
 }
 
 IntArgs make_args(Ctx& c, double dt) {
-  IntArgs a;
+  IntArgs a{};
   a.x = c.x[c.cur];
   a.v = c.v[c.cur];
   a.f32 = c.f32;
   a.f64 = c.f64;
   a.type = c.type[c.cur];
   a.flag = c.flag[c.cur];
-  a.cellid = c.cellid;
   a.q = c.q;
   a.qb = c.qb;
   a.xb = c.xb;
@@ -202,12 +304,53 @@ IntArgs make_args(Ctx& c, double dt) {
     a.hdt_over_m[t] = a.hdt / c.mass[t];
   }
   for (int d = 0; d < 3; d++) {
-    a.M[d] = c.grid.M[d];
     a.origin[d] = c.grid.origin[d];
     a.h[d] = c.grid.h[d];
   }
   a.half_skin2 = (float)(0.25 * c.skin * c.skin);
+  a.clamp_axes = c.clamp_axes;
+  a.damp = c.damp_factor;
+  a.range_on = c.range_border >= 0.0;
+  if (c.nrep > 0) {
+    a.rep = c.rep;
+    a.rep_dt = c.rep_dt;
+    a.rep_grid = c.rep_grid;
+    a.rep_v2 = c.rep_v2;
+    a.rep_d2 = c.rep_d2;
+    a.range_lo[0] = c.range_border;  // per-replica box widened in the kernel
+  } else {
+    for (int d = 0; d < 3; d++) {
+      a.range_lo[d] = c.lo[d] - c.range_border;
+      a.range_hi[d] = c.hi[d] + c.range_border;
+    }
+  }
   return a;
+}
+
+template <bool FP64F, bool FLAGS, bool BATCH>
+void launch_int(Ctx& c, const IntArgs& a, bool drift) {
+  const int nb = grid_blocks(c.n, kBlock);
+  if (drift) k_kick_drift<FP64F, FLAGS, BATCH><<<nb, kBlock, 0, c.stream>>>(a);
+  else k_kick<FP64F, FLAGS, BATCH><<<nb, kBlock, 0, c.stream>>>(a);
+}
+
+void dispatch_int(Ctx& c, double dt, bool drift, bool first = false) {
+  IntArgs a = make_args(c, dt);
+  a.first = first ? 1 : 0;
+  const bool batch = c.nrep > 0;
+  const bool flags = c.has_flags;
+  if (c.fp64) {
+    if (batch) throw ApiFail(OMDG_EINVAL, "replicas run with FP32 forces");
+    if (flags) launch_int<true, true, false>(c, a, drift);
+    else launch_int<true, false, false>(c, a, drift);
+  } else if (batch) {
+    if (flags) launch_int<false, true, true>(c, a, drift);
+    else launch_int<false, false, true>(c, a, drift);
+  } else {
+    if (flags) launch_int<false, true, false>(c, a, drift);
+    else launch_int<false, false, false>(c, a, drift);
+  }
+  OMDG_HIP(hipGetLastError());
 }
 
 // ---- reductions -------------------------------------------------------------
@@ -293,31 +436,11 @@ __global__ void __launch_bounds__(kBlock) k_energy_final(const double* part, int
 
 }  // namespace
 
-void integrate_kick_drift(Ctx& c, double dt) {
-  IntArgs a = make_args(c, dt);
-  const int nb = grid_blocks(c.n, kBlock);
-  if (c.fp64) {
-    if (c.has_flags) k_kick_drift<true, true><<<nb, kBlock, 0, c.stream>>>(a);
-    else k_kick_drift<true, false><<<nb, kBlock, 0, c.stream>>>(a);
-  } else {
-    if (c.has_flags) k_kick_drift<false, true><<<nb, kBlock, 0, c.stream>>>(a);
-    else k_kick_drift<false, false><<<nb, kBlock, 0, c.stream>>>(a);
-  }
-  OMDG_HIP(hipGetLastError());
-}
+void integrate_kick_drift(Ctx& c, double dt) { dispatch_int(c, dt, true); }
 
-void integrate_kick(Ctx& c, double dt) {
-  IntArgs a = make_args(c, dt);
-  const int nb = grid_blocks(c.n, kBlock);
-  if (c.fp64) {
-    if (c.has_flags) k_kick<true, true><<<nb, kBlock, 0, c.stream>>>(a);
-    else k_kick<true, false><<<nb, kBlock, 0, c.stream>>>(a);
-  } else {
-    if (c.has_flags) k_kick<false, true><<<nb, kBlock, 0, c.stream>>>(a);
-    else k_kick<false, false><<<nb, kBlock, 0, c.stream>>>(a);
-  }
-  OMDG_HIP(hipGetLastError());
-}
+void integrate_kick(Ctx& c, double dt) { dispatch_int(c, dt, false); }
+
+void integrate_first(Ctx& c) { dispatch_int(c, 0.0, false, true); }
 
 void integrate_status(Ctx& c) {
   k_status_max<<<1, kBlock, 0, c.stream>>>(c.dmax, grid_blocks(c.n, kBlock), c.dstat);

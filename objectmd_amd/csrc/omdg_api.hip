@@ -260,20 +260,61 @@ static void raise_device_errors(Ctx& c) {
 }
 
 // (Re)build the neighbour list; sizes the list on the first build.
+static void drop_graph(Ctx& c) {
+  if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
+  c.graph_exec = nullptr;
+  c.graph_steps = 0;
+}
+
+// List radius per type pair for replicas: the interaction range of the pair
+// plus the skin; pairs that do not interact get none.
+static void batch_radii(Ctx& c) {
+  for (int i = 0; i < kMaxTypes; i++)
+    for (int j = 0; j < kMaxTypes; j++) {
+      double cut = 0.0;
+      const int tp = c.tab_index[OMDG_TAB_PAIR][i][j];
+      bool any_tab = false;
+      for (auto& row : c.tab_index[OMDG_TAB_PAIR])
+        for (int t : row) any_tab |= t >= 0;
+      if (c.potential == OMDG_POT_SW) {
+        if (!any_tab || (i == 0 && j == 0)) cut = (double)c.sw.a * (double)c.sw.sigma;
+        if (tp >= 0 && !(i == 0 && j == 0)) {
+          const DevTable& t = c.tables[tp];
+          cut = t.r0 + (double)(t.n - 1) * t.dr;
+        }
+      } else {
+        cut = c.cut;
+      }
+      const double rl = cut > 0.0 ? cut + c.skin : 0.0;
+      c.rl2_type[i * kMaxTypes + j] = (float)(rl * rl);
+    }
+}
+
 static void rebuild(Ctx& c) {
   c.rlist = c.cut + c.skin;
   if (c.maxnbr == 0) {
     c.maxnbr = c.maxnbr_req > 0 ? c.maxnbr_req : 64;
+  }
+  if (c.nrep > 0) {
+    if (c.fp64 || c.potential != OMDG_POT_SW)
+      throw ApiFail(OMDG_EINVAL, "replicas run the FP32 Stillinger-Weber kernel (with pair tables)");
+    // grids from the current positions and list radius; the captured steps
+    // refer to the old grid arrays
+    drop_graph(c);
+    batch_radii(c);
+    batch_setup_grids(c, c.rep_lo.data(), c.rep_hi.data(), c.rep_pbc.data());
   }
   for (int attempt = 0; attempt < 3; attempt++) {
     size_t need = (size_t)c.maxnbr * (size_t)c.n;
     if ((long)need > c.cap_nbr) {
       dalloc(c.nbr, need);
       c.cap_nbr = (long)need;
+      drop_graph(c);
     }
     {
       TimerScope ts(c, TG_BUILD);
-      nbr_build(c);
+      if (c.nrep > 0) batch_build(c, true);
+      else nbr_build(c);
     }
     read_status(c);
     int got = c.hstat[ST_MAXNBR];
@@ -285,7 +326,10 @@ static void rebuild(Ctx& c) {
     if (c.maxnbr_req > 0) break;  // caller fixed the size: report overflow
     int z = 0;
     OMDG_HIP(hipMemcpy(c.dstat + ST_ERR, &z, sizeof(int), hipMemcpyHostToDevice));
-    c.maxnbr = ((got + got / 4 + 15) / 16) * 16;  // 25% head room
+    // head room: 25 %, or 50 % for replicas whose lists are rebuilt inside
+    // captured graphs where they cannot grow
+    const int room = c.nrep > 0 ? got / 2 : got / 4;
+    c.maxnbr = ((got + room + 15) / 16) * 16;
   }
   raise_device_errors(c);
 }
@@ -326,7 +370,13 @@ static void update_cut(Ctx& c) {
         }
       }
   } else if (c.potential == OMDG_POT_SW) {
+    // pair tables of the other type pairs may reach further than SW
     cut = (double)c.sw.a * (double)c.sw.sigma;
+    for (int a = 0; a < kMaxTypes; a++)
+      for (int b = 0; b < kMaxTypes; b++) {
+        int tp = c.tab_index[OMDG_TAB_PAIR][a][b];
+        if (tp >= 0 && (a | b) != 0) cut = std::max(cut, tmax(tp));
+      }
   } else if (c.potential == OMDG_POT_TERSOFF) {
     cut = (double)c.ters.R + (double)c.ters.D;
   }
@@ -421,6 +471,20 @@ int omdg_destroy(omdg_t h) {
   dfree(c.dstat);
   dfree(c.dres);
   dfree(c.aabb);
+  drop_graph(c);
+  dfree(c.rep);
+  dfree(c.rep_offd);
+  dfree(c.rep_grid);
+  dfree(c.rep_h);
+  dfree(c.rep_dt);
+  dfree(c.rep_time);
+  dfree(c.rep_e);
+  dfree(c.rep_v2);
+  dfree(c.dstep);
+  dfree(c.cell_rep);
+  dfree(c.rep_need);
+  dfree(c.rep_d2);
+  dfree(c.rep_vcur);
   for (hipEvent_t e : c.ev_pool) (void)hipEventDestroy(e);
   if (c.hstat) (void)hipHostFree(c.hstat);
   if (c.hres) (void)hipHostFree(c.hres);
@@ -496,6 +560,8 @@ int omdg_upload_atoms(omdg_t h, int n, const double* x3, const double* v3, const
     c.list_valid = false;
     c.forces_valid = false;
     c.eam_shape[0] = c.eam_shape[1] = -1;
+    c.nrep = 0;  // a new atom set: replicas are declared again
+    drop_graph(c);
   });
 }
 
@@ -742,11 +808,24 @@ int omdg_neighbor_export(omdg_t h, int maxn, int* cnt, int* code) {
     OMDG_HIP(hipMemcpy(nn.data(), c.nnbr, sizeof(int) * n, hipMemcpyDeviceToHost));
     OMDG_HIP(hipMemcpy(cell.data(), c.cellid, sizeof(int) * n, hipMemcpyDeviceToHost));
     OMDG_HIP(hipMemcpy(lst.data(), c.nbr, sizeof(uint32_t) * lst.size(), hipMemcpyDeviceToHost));
-    const Grid& g = c.grid;
+    std::vector<int> rep;
+    if (c.nrep > 0) {
+      rep.resize(n);
+      OMDG_HIP(hipMemcpy(rep.data(), c.rep, sizeof(int) * n, hipMemcpyDeviceToHost));
+    }
     bool ovf = false;
     for (int s = 0; s < n; s++) {
       int i = id[s];
-      int k[3] = {cell[s] % g.M[0], (cell[s] / g.M[0]) % g.M[1], cell[s] / (g.M[0] * g.M[1])};
+      // the cell grid of the atom's system (its replica's with replicas)
+      Grid g = c.grid;
+      int cl = cell[s];
+      if (c.nrep > 0) {
+        const RepGrid& rg = c.rep_grid_h[rep[s]];
+        for (int d = 0; d < 3; d++) g.M[d] = rg.M[d];
+        g.pbc = rg.pbc;
+        cl -= (int)rg.cell_off;
+      }
+      int k[3] = {cl % g.M[0], (cl / g.M[0]) % g.M[1], cl / (g.M[0] * g.M[1])};
       std::vector<int> row;
       row.reserve(nn[s]);
       for (int e = 0; e < nn[s]; e++) {
@@ -810,6 +889,7 @@ int omdg_step(omdg_t h, int nsteps, double dt, double* ekin, double* epot) {
     Ctx& c = h->c;
     OMDG_HIP(hipSetDevice(c.device));
     check_ready(c);
+    if (c.nrep > 0) throw ApiFail(OMDG_EINVAL, "replicas advance with omdg_batch_run");
     ensure_list(c);
     if (!c.forces_valid) {
       // the first half kick needs forces of the current positions
@@ -852,6 +932,167 @@ int omdg_step(omdg_t h, int nsteps, double dt, double* ekin, double* epot) {
     raise_device_errors(c);
     if (ekin) *ekin = c.ekin;
     if (epot) *epot = c.epot;
+  });
+}
+
+int omdg_set_modifies(omdg_t h, double dt_max, double maxpath, int update, int clamp_axes,
+                      double damp_factor, double range_border) {
+  if (!h || update < 1 || clamp_axes < 0 || clamp_axes > 7) {
+    set_error("bad modify parameters");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    c.dyn_dtmax = dt_max;
+    c.dyn_maxpath = maxpath;
+    c.dyn_update = update;
+    c.clamp_axes = clamp_axes;
+    c.damp_factor = damp_factor;
+    c.range_border = range_border;
+    if (maxpath > 0 && !(dt_max > 0)) throw ApiFail(OMDG_EINVAL, "dynamic time step needs dt_max");
+    drop_graph(c);
+  });
+}
+
+int omdg_set_replicas(omdg_t h, int nrep, const int* off, const double* lo3, const double* hi3,
+                      const int* pbc) {
+  if (!h || nrep < 1 || !off || !lo3 || !hi3 || !pbc) {
+    set_error("bad replica arguments");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    if (c.n <= 0) throw ApiFail(OMDG_EINVAL, "upload the atoms first");
+    if (off[0] != 0 || off[nrep] != c.n) throw ApiFail(OMDG_EINVAL, "replica offsets must cover 0..n");
+    for (int r = 0; r < nrep; r++)
+      if (off[r + 1] <= off[r]) throw ApiFail(OMDG_EINVAL, "empty replica");
+    if (c.list_valid && c.cur != 0) throw ApiFail(OMDG_EINVAL, "set replicas before the first list build");
+    c.nrep = nrep;
+    c.rep_off.assign(off, off + nrep + 1);
+    c.rep_lo.assign(lo3, lo3 + 3 * (size_t)nrep);
+    c.rep_hi.assign(hi3, hi3 + 3 * (size_t)nrep);
+    c.rep_pbc.assign(pbc, pbc + nrep);
+    std::vector<int> rid(c.n);
+    for (int r = 0; r < nrep; r++)
+      for (int i = off[r]; i < off[r + 1]; i++) rid[i] = r;
+    dalloc(c.rep, c.n);
+    dalloc(c.rep_offd, nrep + 1);
+    dalloc(c.rep_dt, nrep);
+    dalloc(c.rep_time, nrep);
+    dalloc(c.rep_e, 2 * (size_t)nrep);
+    dalloc(c.rep_v2, nrep);
+    dalloc(c.rep_vcur, nrep);
+    dalloc(c.rep_d2, nrep);
+    dalloc(c.rep_need, nrep);
+    if (!c.dstep) dalloc(c.dstep, 1);
+    OMDG_HIP(hipMemcpy(c.rep, rid.data(), sizeof(int) * c.n, hipMemcpyHostToDevice));
+    OMDG_HIP(hipMemcpy(c.rep_offd, off, sizeof(int) * (nrep + 1), hipMemcpyHostToDevice));
+    std::vector<double> dt(nrep, c.dyn_dtmax > 0 ? c.dyn_dtmax : 0.001), zero(2 * (size_t)nrep, 0.0);
+    OMDG_HIP(hipMemcpy(c.rep_dt, dt.data(), sizeof(double) * nrep, hipMemcpyHostToDevice));
+    OMDG_HIP(hipMemcpy(c.rep_time, zero.data(), sizeof(double) * nrep, hipMemcpyHostToDevice));
+    OMDG_HIP(hipMemcpy(c.rep_e, zero.data(), sizeof(double) * 2 * nrep, hipMemcpyHostToDevice));
+    OMDG_HIP(hipMemset(c.dstep, 0, sizeof(int)));
+    OMDG_HIP(hipMemset(c.rep_v2, 0, sizeof(unsigned long long) * nrep));
+    OMDG_HIP(hipMemset(c.rep_vcur, 0, sizeof(double) * nrep));
+    OMDG_HIP(hipMemset(c.rep_d2, 0, sizeof(unsigned int) * nrep));
+    OMDG_HIP(hipMemset(c.rep_need, 0, sizeof(int) * nrep));
+    c.rep_steps = 0;
+    c.list_valid = false;
+    c.forces_valid = false;
+    drop_graph(c);
+  });
+}
+
+// One step of every replica: no host synchronisation, so it can be captured.
+static void batch_one_step(Ctx& c, bool energy) {
+  batch_rep_dt(c);
+  integrate_kick_drift(c, 0.0);
+  batch_rep_need(c);
+  batch_build(c, false);
+  compute_forces(c, energy);
+  integrate_kick(c, 0.0);
+  batch_rep_tick(c);
+}
+
+int omdg_batch_run(omdg_t h, int nsteps, double* time, double* ekin, double* epot) {
+  if (!h || nsteps < 0) {
+    set_error("bad argument to omdg_batch_run");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    OMDG_HIP(hipSetDevice(c.device));
+    check_ready(c);
+    if (c.nrep == 0) throw ApiFail(OMDG_EINVAL, "no replicas (omdg_set_replicas)");
+    if (c.fp64 || c.potential != OMDG_POT_SW)
+      throw ApiFail(OMDG_EINVAL, "replicas run the FP32 Stillinger-Weber kernel (with pair tables)");
+    if (!(c.dyn_maxpath > 0) && !(c.dyn_dtmax > 0))
+      throw ApiFail(OMDG_EINVAL, "replicas need a time step (omdg_set_modifies)");
+    ensure_list(c);
+    if (c.rep_steps == 0) {
+      // MDSystem::FirstRun: forces with their modifies, RangeLimiter, the
+      // speed maxima for the first DynamicTimeStep update, zero forces
+      compute_forces(c, true);
+      integrate_first(c);
+      batch_rep_vinit(c);
+      zero_forces(c);
+      c.forces_valid = true;
+    } else if (!c.forces_valid) {
+      compute_forces(c, false);
+    }
+    // the list rebuilds are decided on the device, so a graph holds any
+    // number of steps: check_every steps per replay; the last step runs
+    // outside the graph and evaluates the energies (MeasurePotential)
+    const int K = std::max(1, c.check_every);
+    int done = 0;
+    while (nsteps - done > K) {
+      if (!c.graph_exec || c.graph_steps != K) {
+        drop_graph(c);
+        const bool was_timing = c.timing;
+        c.timing = false;
+        hipGraph_t g = nullptr;
+        OMDG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        try {
+          for (int j = 0; j < K; j++) batch_one_step(c, false);
+        } catch (...) {
+          (void)hipStreamEndCapture(c.stream, &g);
+          if (g) (void)hipGraphDestroy(g);
+          c.timing = was_timing;
+          throw;
+        }
+        OMDG_HIP(hipStreamEndCapture(c.stream, &g));
+        const hipError_t e = hipGraphInstantiate(&c.graph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        c.timing = was_timing;
+        if (e != hipSuccess) {
+          c.graph_exec = nullptr;
+          throw HipFail(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        c.graph_steps = K;
+      }
+      {
+        TimerScope ts(c, TG_GRAPH);
+        OMDG_HIP(hipGraphLaunch(c.graph_exec, c.stream));
+      }
+      done += K;
+    }
+    for (; done < nsteps; done++) batch_one_step(c, done == nsteps - 1);
+    c.rep_steps += nsteps;
+    // Ek after the last kick, Ep of the last force pass (FirstRun's before
+    // any step)
+    batch_rep_energy(c);
+    std::vector<double> t(c.nrep), e(2 * (size_t)c.nrep);
+    OMDG_HIP(hipMemcpyAsync(t.data(), c.rep_time, sizeof(double) * c.nrep, hipMemcpyDeviceToHost,
+                            c.stream));
+    OMDG_HIP(hipMemcpyAsync(e.data(), c.rep_e, sizeof(double) * 2 * c.nrep, hipMemcpyDeviceToHost,
+                            c.stream));
+    read_status(c);
+    raise_device_errors(c);
+    for (int r = 0; r < c.nrep; r++) {
+      if (time) time[r] = t[r];
+      if (ekin) ekin[r] = e[2 * r];
+      if (epot) epot[r] = e[2 * r + 1];
+    }
   });
 }
 

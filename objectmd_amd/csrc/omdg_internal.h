@@ -23,6 +23,7 @@ enum : int {
   ST_DISP2 = 1,      // max |x - x_build|^2 since the last build (float bits)
   ST_VMAX2 = 2,      // max |v|^2 seen by the drift (float bits)
   ST_MAXNBR = 3,     // largest neighbour count of the last build
+  ST_REBUILDS = 4,   // replica list rebuilds decided on the device
   ST_COUNT = 8
 };
 enum : int {
@@ -61,6 +62,19 @@ struct Grid {
   double L[3] = {0, 0, 0};
   int pbc = 0;
   long ncell() const { return (long)M[0] * M[1] * M[2]; }
+};
+
+// Cell grid of one replica in a batch; its cells are [cell_off, cell_off +
+// M0 M1 M2) of the global cell range.
+struct RepGrid {
+  int M[3];
+  int pbc;
+  double lo[3], hi[3];  // the replica's box
+  double origin[3];
+  double h[3];
+  double L[3];
+  double inv_h[3];
+  long cell_off;
 };
 
 struct SwParams {
@@ -155,6 +169,37 @@ struct Ctx {
   // results of the last evaluation
   double epot = 0, virial = 0, ekin = 0;
   bool forces_valid = false;
+  // independent replicas (omdg_set_replicas); atoms keep the caller's order
+  int nrep = 0;
+  std::vector<int> rep_off;  // host, nrep + 1
+  std::vector<double> rep_lo, rep_hi;
+  std::vector<int> rep_pbc;
+  std::vector<RepGrid> rep_grid_h;
+  int* rep = nullptr;          // device: replica of each atom
+  int* rep_offd = nullptr;     // device copy of rep_off
+  RepGrid* rep_grid = nullptr; // device
+  float4* rep_h = nullptr;     // device: cell edges per replica
+  double* rep_dt = nullptr;    // device: time step per replica
+  double* rep_time = nullptr;  // device: elapsed time per replica
+  double* rep_e = nullptr;     // device: Ek, Ep per replica
+  unsigned long long* rep_v2 = nullptr;  // device: max |v|^2 per replica (bits)
+  long rep_steps = 0;          // steps run since omdg_set_replicas
+  int* dstep = nullptr;        // device step counter (DynamicTimeStep cadence)
+  long rep_ncell = 0;
+  int* cell_rep = nullptr;     // device: replica of each cell
+  int* rep_need = nullptr;     // device: rebuild replica r in this step
+  unsigned int* rep_d2 = nullptr;  // device: max displacement^2 (float bits)
+  double* rep_vcur = nullptr;  // device: max |v|^2 at the start of the step
+  float rl2_type[kMaxTypes * kMaxTypes] = {0};  // list radius^2 per type pair
+  // per-step modifies (omdg_set_modifies)
+  double dyn_dtmax = 0.0, dyn_maxpath = 0.0;
+  int dyn_update = 1;
+  int clamp_axes = 7;
+  double damp_factor = 0.0;
+  double range_border = -1.0;
+  // captured steps of the batch loop
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_steps = 0;
   // kernel timing (omdg_timing)
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
@@ -163,7 +208,7 @@ struct Ctx {
 };
 
 enum : int { TG_BUILD = 0, TG_PASS1 = 1, TG_FORCE = 2, TG_DRIFT = 3, TG_KICK = 4,
-             TG_REDUCE = 5, TG_COUNT = 6 };
+             TG_REDUCE = 5, TG_GRAPH = 6, TG_COUNT = 7 };
 
 // Brackets the launches of one group with HIP events when timing is on.
 struct TimerScope {
@@ -216,7 +261,18 @@ void sw_forces(Ctx& c, bool energy);
 void tersoff_forces(Ctx& c, bool energy);
 void integrate_kick_drift(Ctx& c, double dt);
 void integrate_kick(Ctx& c, double dt);
+// the POST_FORCE and POST_INTEGRATION modifies of MDSystem::FirstRun
+// (system.cpp:860-873) without moving anything
+void integrate_first(Ctx& c);
 void integrate_status(Ctx& c);  // fold the drift's block maxima into dstat
+// replicas (batch.hip)
+void batch_setup_grids(Ctx& c, const double* lo3, const double* hi3, const int* pbc);
+void batch_build(Ctx& c, bool all);  // all replicas, or those flagged in rep_need
+void batch_rep_need(Ctx& c);
+void batch_rep_dt(Ctx& c);
+void batch_rep_vinit(Ctx& c);
+void batch_rep_tick(Ctx& c);
+void batch_rep_energy(Ctx& c);
 void reduce_energy(Ctx& c, bool kinetic, bool potential);
 void zero_forces(Ctx& c);
 

@@ -42,6 +42,9 @@ struct TBArgs {
   int* err;
   int n;
   float hx, hy, hz;
+  // independent replicas: each atom's replica and the cell edges of its grid
+  const int* rep;        // null for one system
+  const float4* rep_h;
   float rc, rc2;
   SwParams sw;
   TersoffParams ts;
@@ -122,14 +125,20 @@ __device__ inline void sw_two(const TBArgs& a, float r, float& fr, float& ehalf)
   }
 }
 
+// stencil offset o * h of image code c (c = (ox+1) + 3 (oy+1) + 9 (oz+1))
+__device__ inline float4 off_of(uint32_t code, const float4& h) {
+  const int c = (int)code;
+  return make_float4((float)(c % 3 - 1) * h.x, (float)((c / 3) % 3 - 1) * h.y,
+                     (float)(c / 9 - 1) * h.z, 0.f);
+}
+
 template <bool ENERGY>
 __global__ void __launch_bounds__(kBlock) k_sw(TBArgs a) {
-  __shared__ float4 offs[27];
-  load_offsets(offs, a.hx, a.hy, a.hz);
-  __syncthreads();
   const int l = xcd_block() * blockDim.x + threadIdx.x;
   if (l >= a.n) return;
   const int n = a.n;
+  // neighbours and their centres belong to the same replica
+  const float4 hh = a.rep ? a.rep_h[a.rep[l]] : make_float4(a.hx, a.hy, a.hz, 0.f);
   const float4 ql = a.q[l];
   const int tl = snap_type(ql.w);
   const bool l_sw = a.sw_type < 0 || tl == a.sw_type;
@@ -142,7 +151,7 @@ __global__ void __launch_bounds__(kBlock) k_sw(TBArgs a) {
   for (int k = 0; k < cnt; k++) {
     const uint32_t e = a.nbr[nidx(l, k, n)];
     const float4 qj = a.q[e & kSlotMask];
-    const float3 d = disp(ql, qj, offs[e >> kImgShift]);  // x_j - x_l
+    const float3 d = disp(ql, qj, off_of(e >> kImgShift, hh));  // x_j - x_l
     const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
     if (r2 < 1e-6f) {
       atomicOr(a.err, ERR_OVERLAP);
@@ -210,7 +219,7 @@ __global__ void __launch_bounds__(kBlock) k_sw(TBArgs a) {
       const int t = (int)(e & kSlotMask);
       const float4 qk = a.q[t];
       if (a.sw_type >= 0 && snap_type(qk.w) != a.sw_type) continue;
-      const float3 d = disp(qc, qk, offs[e >> kImgShift]);  // x_k - x_c
+      const float3 d = disp(qc, qk, off_of(e >> kImgShift, hh));  // x_k - x_c
       const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
       if (!(r2 < a.rc2)) continue;
       if (t == l) {
@@ -492,6 +501,10 @@ TBArgs base_args(Ctx& c) {
   a.hx = (float)c.grid.h[0];
   a.hy = (float)c.grid.h[1];
   a.hz = (float)c.grid.h[2];
+  if (c.nrep > 0) {
+    a.rep = c.rep;
+    a.rep_h = c.rep_h;
+  }
   return a;
 }
 

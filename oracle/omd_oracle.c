@@ -932,7 +932,6 @@ void orc_vv_kick_drift(int n, double *xyz, double *vel, const double *f,
     double hdt = 0.5 * dt;
     for (int i = 0; i < n; i++) {
         if (!active(flag, i)) continue;
-        if (flag && (flag[i] & 8)) continue; /* fixed atom */
         double M = mass[i];
         for (int d = 0; d < 3; d++) vel[3 * i + d] += hdt * f[3 * i + d] / M;
         for (int d = 0; d < 3; d++) xyz[3 * i + d] += dt * (vel[3 * i + d]);
@@ -944,7 +943,6 @@ void orc_vv_kick(int n, double *vel, const double *f, const double *mass,
     double hdt = 0.5 * dt;
     for (int i = 0; i < n; i++) {
         if (!active(flag, i)) continue;
-        if (flag && (flag[i] & 8)) continue;
         double M = mass[i];
         for (int d = 0; d < 3; d++) vel[3 * i + d] += hdt * f[3 * i + d] / M;
     }

@@ -260,3 +260,116 @@ def _declare():
 
 
 _declare()
+
+
+# ---------------------------------------------------------------------------
+# impact runs (test/si-impact/main.cpp): SW among type 0, a pair table between
+# type 0 and type 1, and the per-step modifies
+# ---------------------------------------------------------------------------
+FLAG_ACTIVE, FLAG_FIXED, FLAG_DAMP, FLAG_LIMIT = 1, 8, 16, 32
+
+
+def _compact(code, mask):
+    """Rows of code reduced to the entries where mask holds (sorted)."""
+    big = np.iinfo(np.int32).max
+    c2 = np.where(mask, code, big)
+    c2.sort(axis=1)
+    cnt = mask.sum(1).astype(np.int32)
+    c2[np.arange(code.shape[1])[None, :] >= cnt[:, None]] = 0
+    return cnt, np.ascontiguousarray(c2, np.int32)
+
+
+def impact_forces(x, tid, flag, lo, L, pbc, swp, pair, maxn=192):
+    """StillingerWeber among type-0 atoms plus TForcePair between types 0
+    and 1 (test/si-impact/main.cpp:71-72), on lists rebuilt at the
+    interaction range.  Returns forces and per-atom energies."""
+    tid = np.asarray(tid)
+    xw = wrap(x, lo, L, pbc)
+    rc_sw = swp.a * swp.sigma
+    rc_pair = pair.t.r0 + (pair.t.n - 1) * pair.t.dr
+    cnt, code = neighbors(xw, lo, L, pbc, max(rc_sw, rc_pair), maxn=maxn, flag=flag)
+    j = code // 27
+    valid = np.arange(code.shape[1])[None, :] < cnt[:, None]
+    ti = tid[:, None]
+    tj = tid[j]
+    f_sw, p_sw = sw_forces(xw, lo, L, pbc, _compact(code, valid & (ti == 0) & (tj == 0)),
+                           swp, flag)
+    f_pr, p_pr, _ = pair_forces(xw, lo, L, pbc, _compact(code, valid & (ti != tj)), pair,
+                                flag)
+    return f_sw + f_pr, p_sw + p_pr
+
+
+def impact_run(x, v, tid, flag, mass_by_type, lo, hi, pbc, swp, pair, nsteps, dt_max,
+               maxpath, update, clamp_axes, damp, range_border, maxn=192):
+    """One impact system through MDSystem::FirstRun (system.cpp:860-873) and
+    nsteps of RunKernel (system.cpp:964-983):
+      PreIntegration  DynamicTimeStep every `update` steps,
+                      dt = min(dt_max, maxpath / (1.1 vmax)), dt_max at
+                      vmax = 0 (include/modify/DynamicTimeStep.h:61-71)
+      Integrate       velocity Verlet (integrator.cpp:225-259) on active
+                      atoms; Iterate's POST_FORCE modifies ForceDamper
+                      f -= damp v (ForceDamper.h:39-46) and CoordClamp
+                      f = v = 0 on the clamped axes (CoordClamp.h:50-56),
+                      on every atom of their groups
+      MeasureKinetic  SqrMaxVelocity over active atoms (system.cpp:1077-1094)
+      PostIntegration RangeLimiter: LIMIT atoms outside the box widened by
+                      range_border are deactivated (RangeLimiter.h:54-65)
+    FirstRun: forces with their modifies, RangeLimiter, MeasureKinetic, then
+    zero forces.  Returns dict x, v, flag, dts (per step), time, ek, ep."""
+    x = np.ascontiguousarray(x, np.float64).copy()
+    v = np.ascontiguousarray(v, np.float64).copy()
+    flag = np.ascontiguousarray(flag, np.int32).copy()
+    tid = np.asarray(tid, np.int32)
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    L = hi - lo
+    m = np.asarray(mass_by_type, np.float64)[tid]
+    lo_r = lo - range_border
+    hi_r = hi + range_border
+
+    def post_force(f):
+        dm = (flag & FLAG_DAMP) != 0
+        if damp != 0.0:
+            f[dm] = f[dm] - damp * v[dm]
+        fx = (flag & FLAG_FIXED) != 0
+        for ax in range(3):
+            if (clamp_axes >> ax) & 1:
+                f[fx, ax] = 0.0
+                v[fx, ax] = 0.0
+
+    def sqr_max_velocity():
+        act = (flag & FLAG_ACTIVE) != 0
+        return float((v[act] ** 2).sum(1).max()) if act.any() else 0.0
+
+    def range_limit():
+        if range_border < 0.0:
+            return
+        lim = ((flag & FLAG_LIMIT) != 0) & ((flag & FLAG_ACTIVE) != 0)
+        out = ((x < lo_r) | (x > hi_r)).any(1)
+        flag[lim & out] &= ~FLAG_ACTIVE
+
+    f, pot = impact_forces(x, tid, flag, lo, L, pbc, swp, pair, maxn)
+    post_force(f)
+    range_limit()
+    vmax2 = sqr_max_velocity()
+    f[:] = 0.0
+    dt = dt_max
+    dts = []
+    for step in range(nsteps):
+        if maxpath > 0.0 and step % update == 0:
+            dt = dt_max
+            if vmax2 > 0.0:
+                dt = maxpath / (np.sqrt(vmax2) * 1.10)
+                if dt > dt_max:
+                    dt = dt_max
+        vv_kick_drift(x, v, f, m, dt, flag)
+        f, pot = impact_forces(x, tid, flag, lo, L, pbc, swp, pair, maxn)
+        post_force(f)
+        vv_kick(v, f, m, dt, flag)
+        vmax2 = sqr_max_velocity()
+        range_limit()
+        dts.append(dt)
+    act = (flag & FLAG_ACTIVE) != 0
+    ek = 0.5 * float((m[act, None] * v[act] ** 2).sum())
+    return dict(x=x, v=v, flag=flag, dts=np.array(dts), time=float(np.sum(dts)), ek=ek,
+                ep=float(pot.sum()))

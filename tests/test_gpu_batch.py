@@ -1,0 +1,191 @@
+"""GPU parity of the batched impact replicas (omdg_set_replicas /
+omdg_batch_run) against the FP64 oracle run one replica at a time:
+neighbour lists bit-exact per replica, forces, whole runs with
+DynamicTimeStep / ForceDamper / CoordClamp / RangeLimiter, and graph
+replays against the same steps launched one by one."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import objectmd_amd as om
+from objectmd_amd import impact
+from oracle import oracle as O
+from tests.common import GOLDEN, rel_rms
+
+pytestmark = pytest.mark.gpu
+SW_PATH = os.path.join(GOLDEN, "ref", "sw.silicon")
+PAIR_PATH = os.path.join(GOLDEN, "ref", "pair.ar-si.zbl")
+FORCE_RTOL = 1e-5
+SKIN = 0.5
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def oracle_pot():
+    y, dr, r0, rm = O.table_load(PAIR_PATH, "")
+    p = impact.sw_params(SW_PATH)
+    return O.SwParam(*p[:10], int(p[10]), *p[11:]), O.Spline(y, dr, r0, rm)
+
+
+def small_batch(seed, sizes=(300, 700, 1200), energies=(120.0, 300.0, 60.0),
+                thetas=(0.0, 35.0, 55.0), height=None):
+    """Replicas with 0.05 A displacements off the ideal slab, so that the
+    forces on every atom are of bond size (an ideal lattice has FP64 forces
+    near zero, which FP32 sums only reach to ~1e-5 eV/A)."""
+    reps = []
+    rng = np.random.default_rng(seed)
+    for r, (n, e, t) in enumerate(zip(sizes, energies, thetas)):
+        p = impact.replica(n, e, t, 40.0 * r, seed + r)
+        p["x"][:-1] += rng.normal(0.0, 0.05, p["x"][:-1].shape)
+        if height is not None:
+            p["x"][-1, 2] = height
+        reps.append(p)
+    off = np.zeros(len(reps) + 1, np.int32)
+    off[1:] = np.cumsum([len(p["x"]) for p in reps])
+    return dict(x=np.vstack([p["x"] for p in reps]), v=np.vstack([p["v"] for p in reps]),
+                tid=np.concatenate([p["tid"] for p in reps]),
+                flag=np.concatenate([p["flag"] for p in reps]), off=off,
+                lo=np.array([p["lo"] for p in reps]), hi=np.array([p["hi"] for p in reps]),
+                pbc=np.array([3] * len(reps), np.int32),
+                mass=[om.tables.mass_sim(om.tables.MASS_SI),
+                      om.tables.mass_sim(om.tables.MASS_AR)])
+
+
+def engine(b, graph_steps=10):
+    eng = om.Engine()
+    impact.setup_engine(eng, b, SW_PATH, PAIR_PATH, skin=SKIN, graph_steps=graph_steps)
+    return eng
+
+
+def wrap_rep(x, lo, hi):
+    return O.wrap(x, lo, hi - lo, 3)
+
+
+def test_batch_lists_bit_exact():
+    b = small_batch(11, height=2.5)
+    say(f"[batch-list] n={len(b['x'])} nrep={len(b['off']) - 1}")
+    eng = engine(b)
+    eng.neighbor_build()
+    cnt, code = eng.neighbor_export(128)
+    swp, pair = oracle_pot()
+    f32 = lambda v: float(np.float32(v))
+    rc_sw = f32(swp.a) * f32(swp.sigma)
+    rc_pair = pair.t.r0 + (pair.t.n - 1) * pair.t.dr
+    rl2 = {(0, 0): f32((rc_sw + SKIN) ** 2), (0, 1): f32((rc_pair + SKIN) ** 2),
+           (1, 0): f32((rc_pair + SKIN) ** 2), (1, 1): 0.0}
+    for r in range(len(b["off"]) - 1):
+        i0, i1 = b["off"][r], b["off"][r + 1]
+        lo, hi = b["lo"][r], b["hi"][r]
+        L = hi - lo
+        x = wrap_rep(b["x"][i0:i1], lo, hi)
+        tid = b["tid"][i0:i1]
+        oc, ocode = O.neighbors(x, lo, L, 3, (rc_pair + SKIN) * (1 + 1e-6), maxn=160)
+        for i in range(i1 - i0):
+            keep = []
+            for c in ocode[i, :oc[i]]:
+                j, img = divmod(int(c), 27)
+                s = (img % 3 - 1, img // 3 % 3 - 1, img // 9 - 1)
+                d = [x[i, k] - (x[j, k] if s[k] == 0 else
+                                (x[j, k] + L[k] if s[k] > 0 else x[j, k] + (-L[k])))
+                     for k in range(3)]
+                r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2]
+                if r2 <= rl2[(tid[i], tid[j])]:
+                    keep.append((j + i0) * 27 + img)
+            got = sorted(code[i0 + i, :cnt[i0 + i]].tolist())
+            assert got == sorted(keep), f"replica {r} atom {i}"
+    eng.close()
+
+
+def test_batch_forces_vs_oracle():
+    b = small_batch(21, height=2.0)
+    say(f"[batch-forces] n={len(b['x'])}")
+    eng = engine(b)
+    eng.compute()
+    f = eng.download(om.F)
+    pot = eng.download(om.POT)
+    swp, pair = oracle_pot()
+    fo = np.zeros_like(f)
+    po = np.zeros(len(f))
+    for r in range(len(b["off"]) - 1):
+        i0, i1 = b["off"][r], b["off"][r + 1]
+        lo, hi = b["lo"][r], b["hi"][r]
+        fo[i0:i1], po[i0:i1] = O.impact_forces(b["x"][i0:i1], b["tid"][i0:i1],
+                                               b["flag"][i0:i1], lo, hi - lo, 3, swp, pair)
+        err = rel_rms(f[i0:i1], fo[i0:i1])
+        say(f"[batch-forces] replica {r} relRMS={err:.3e}")
+        assert err <= FORCE_RTOL
+        # the projectile feels the surface
+        assert np.abs(fo[i1 - 1]).max() > 0.1
+        assert np.abs(f[i1 - 1] - fo[i1 - 1]).max() <= 1e-5 * np.abs(fo[i1 - 1]).max()
+    assert np.abs(pot - po).max() < 2e-5
+    eng.close()
+
+
+def test_batch_run_vs_oracle():
+    """40 steps of every replica (graphs of 10 steps) against the oracle's
+    RunKernel with the impact modifies."""
+    nsteps = 40
+    b = small_batch(31)
+    say(f"[batch-run] n={len(b['x'])} steps={nsteps}")
+    eng = engine(b, graph_steps=10)
+    t, ek, ep = eng.batch_run(nsteps)
+    x = eng.download(om.X)
+    v = eng.download(om.V)
+    swp, pair = oracle_pot()
+    for r in range(len(b["off"]) - 1):
+        i0, i1 = b["off"][r], b["off"][r + 1]
+        lo, hi = b["lo"][r], b["hi"][r]
+        o = O.impact_run(b["x"][i0:i1], b["v"][i0:i1], b["tid"][i0:i1], b["flag"][i0:i1],
+                         b["mass"], lo, hi, 3, swp, pair, nsteps, impact.DT_MAX,
+                         impact.MAXPATH, impact.UPDATE, 7, impact.DAMP_FACTOR,
+                         impact.RANGE_BORDER)
+        dx = wrap_rep(x[i0:i1], lo, hi) - wrap_rep(o["x"], lo, hi)
+        L = hi - lo
+        dx[:, :2] -= L[:2] * np.round(dx[:, :2] / L[:2])
+        say(f"[batch-run] replica {r}: time {t[r]:.6e} / {o['time']:.6e}  "
+            f"max|dx|={np.abs(dx).max():.2e}  ek {ek[r]:.6f}/{o['ek']:.6f}  "
+            f"ep {ep[r]:.6f}/{o['ep']:.6f}")
+        assert t[r] == pytest.approx(o["time"], rel=1e-6)
+        assert np.abs(dx).max() < 1e-4
+        assert rel_rms(v[i0:i1], o["v"]) < 1e-4
+        assert ek[r] == pytest.approx(o["ek"], rel=1e-4)
+        assert ep[r] == pytest.approx(o["ep"], rel=1e-5)
+        # the fixed layers have not moved
+        fixed = (b["flag"][i0:i1] & om.FLAG_FIXED) != 0
+        assert np.array_equal(x[i0:i1][fixed], wrap_rep(b["x"][i0:i1], lo, hi)[fixed])
+    eng.close()
+
+
+def test_graph_replays_match_eager_steps():
+    """The same 30 steps as 5 replays of a 5-step graph plus eager steps,
+    and as eager steps only, give identical states."""
+    b = small_batch(41, sizes=(400, 900), energies=(250.0, 450.0), thetas=(10.0, 50.0))
+    out = []
+    for gs in (5, 1000):
+        eng = engine(b, graph_steps=gs)
+        t, ek, ep = eng.batch_run(30)
+        out.append((eng.download(om.X), eng.download(om.V), t, ek, ep))
+        eng.close()
+    for a, c in zip(out[0], out[1]):
+        assert np.array_equal(a, c)
+
+
+def test_batch_run_in_pieces():
+    """batch_run(20) + batch_run(20) continues exactly like batch_run(40):
+    the forces carried between calls are the modified ones."""
+    b = small_batch(51, sizes=(500,), energies=(200.0,), thetas=(20.0,))
+    eng = engine(b, graph_steps=7)
+    eng.batch_run(20)
+    t1, ek1, ep1 = eng.batch_run(20)
+    x1 = eng.download(om.X)
+    eng.close()
+    eng = engine(b, graph_steps=7)
+    t2, ek2, ep2 = eng.batch_run(40)
+    x2 = eng.download(om.X)
+    eng.close()
+    assert np.array_equal(x1, x2)
+    assert np.array_equal(t1, t2) and np.array_equal(ek1, ek2)
