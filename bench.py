@@ -1,13 +1,25 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X force engine on the EAM aluminium workload.
+"""Benchmark of the MI355X force engine for Object-MD.
 
-Workload: fcc Al, a = 4.05 A, 40^3 cells = 256,000 atoms, periodic,
-300 K Maxwell-Boltzmann velocities plus 0.02 A Gaussian displacements
-(seeded), Johnson-type EAM tabulated to 5,000-point splines with cutoff
-5.5 A, FP32 forces with compensated sums, FP64 velocity Verlet, dt = 2 fs.
-One step = one velocity-Verlet step of all atoms (kick, drift, list check,
-density pass, force pass, kick).  Inputs are resident on the GPU before the
-timed region.
+Default workload (--workload eam): fcc Al, a = 4.05 A, 40^3 cells = 256,000
+atoms, periodic, 300 K Maxwell-Boltzmann velocities plus 0.02 A Gaussian
+displacements (seeded), Johnson-type EAM tabulated to 5,000-point splines
+with cutoff 5.5 A, FP32 forces with compensated sums, FP64 velocity Verlet,
+dt = 2 fs.  One step = one velocity-Verlet step of all atoms (kick, drift,
+list check, density pass, force pass, kick).
+
+Other workloads, same JSON line:
+  lj       Lennard-Jones Ar, fcc a = 5.26 A, 10^3 cells = 4,000 atoms, FP64
+           forces (plain method), 40 K, dt = 1 fs, skin 0.3 A
+  tersoff  Tersoff Si(B) R = 2.7 A, D = 0.15 A, diamond a = 5.431 A, 50^3
+           cells = 1,000,000 atoms, PBC x y with free (001) surfaces,
+           1,000 K velocities, FP32 forces, dt = 1 fs
+  impact   512 Si(001) slab replicas of 2k-20k atoms (log-uniform) at 300 K,
+           each hit by one Ar atom of 50-500 eV at 0-60 degrees, SW + ZBL
+           among Si and the Ar-Si ZBL table, DynamicTimeStep (maxpath
+           0.13 A, dt <= 1 fs), bottom layers clamped, border damped,
+           steps captured in HIP graphs
+Inputs are resident on the GPU before the timed region.
 
 Prints one JSON line: atom-steps/s, the force error of the final state
 against the FP64 oracle, the dominant kernel's HBM roofline fraction
@@ -26,34 +38,26 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import objectmd_amd as om  # noqa: E402
-from objectmd_amd import lattice, tables  # noqa: E402
+from objectmd_amd import impact, lattice, params, tables  # noqa: E402
 
+REF = os.path.join(ROOT, "tests", "golden", "ref")   # parameter files and tables
 HBM_PEAK_GBS = 8000.0
-# compulsory per-atom bytes of one launch of the force pass: read the FP32
-# snapshot (16 B) and F'(rho) (4 B), write the FP32 force (12 B)
-FORCE_PASS_BYTES = 32
-# compulsory per-atom-step bytes of the whole EAM step (integrator 128,
-# density pass 20, force pass 32)
-STEP_BYTES = 180
-
-
-def build_system(cells, seed, disp, temperature, a=4.05):
-    pos, (lo, hi) = lattice.fcc100(cells, cells, cells, a)
-    rng = np.random.default_rng(seed)
-    x = pos + rng.normal(0.0, disp, pos.shape)
-    L = hi - lo
-    for d in range(3):
-        below = x[:, d] < lo[d]
-        x[below, d] = x[below, d] + L[d]
-        above = x[:, d] >= lo[d] + L[d]
-        x[above, d] = x[above, d] + (-L[d])
-    m = tables.mass_sim(tables.MASS_AL)
-    v = lattice.maxwell_boltzmann(len(x), m, temperature, seed + 1)
-    return x, v, lo, hi, m
+METRIC = "atom-steps/sec at stated force error vs FP64 oracle; % HBM roofline"
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def wrap(x, lo, L, pbc):
+    x = x.copy()
+    for d in range(3):
+        if (pbc >> d) & 1:
+            below = x[:, d] < lo[d]
+            x[below, d] = x[below, d] + L[d]
+            above = x[:, d] >= lo[d] + L[d]
+            x[above, d] = x[above, d] + (-L[d])
+    return x
 
 
 def copy_probe(mbytes):
@@ -76,14 +80,310 @@ def copy_probe(mbytes):
     log(f"[bench] copy probe: {nbytes} bytes device to device")
 
 
+def timed_steps(eng, run):
+    eng.timing_start()
+    t0 = time.perf_counter()
+    res = run()                        # returns after the stream drained
+    elapsed = time.perf_counter() - t0
+    return res, elapsed, eng.timing_stop()
+
+
+def kernel_rate(times, group, bytes_per_launch):
+    ms, launches = times[group]
+    avg_s = ms / max(launches, 1) / 1e3
+    return (bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0), avg_s
+
+
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(path):
+        with open(path) as fl:
+            return json.load(fl).get("traffic_bytes_per_launch")
+    return None
+
+
+# ---------------------------------------------------------------------------
+def bench_eam(args):
+    dt, seed = 0.002, 12346
+    cells = args.cells or 40
+    pos, (lo, hi) = lattice.fcc100(cells, cells, cells, 4.05)
+    rng = np.random.default_rng(seed)
+    L = hi - lo
+    x = wrap(pos + rng.normal(0.0, 0.02, pos.shape), lo, L, 7)
+    m = tables.mass_sim(tables.MASS_AL)
+    v = lattice.maxwell_boltzmann(len(x), m, 300.0, seed + 1)
+    n = len(x)
+    skin = 0.6 if args.skin is None else args.skin
+    tab = tables.JohnsonEAM().tables(5000)
+    steps = 2000 if args.steps is None else args.steps
+    warmup = 50 if args.warmup is None else args.warmup
+    log(f"[bench] EAM Al n={n} skin={skin} steps={steps} warmup={warmup}")
+    eng = om.Engine()
+    eng.set_box(lo, hi, 7)
+    eng.upload_atoms(x, v, mass=[m])
+    for kind, key in ((om.TAB_PAIR, "PAIR"), (om.TAB_EDENS, "EDENS"), (om.TAB_EMBED, "EMBED")):
+        eng.add_table(kind, 0, 0, tab[key].y, tab[key].dr, tab[key].r0)
+    eng.set_potential(om.POT_EAM)
+    eng.set_neighbor(skin, args.check_every or 10, 0)
+    base, _ = eng.compute()            # FirstRun: BasePotential, then f = 0
+    eng.zero_forces()
+    e_start = 0.5 * m * float((v * v).sum()) / n
+    if warmup:
+        eng.step(warmup, dt)
+    (ek, ep), elapsed, times = timed_steps(eng, lambda: eng.step(steps, dt))
+    rate = n * steps / elapsed
+    log(f"[bench] {steps} steps in {elapsed:.4f} s: {rate:.4e} atom-steps/s")
+    if args.copy_probe_mb:
+        copy_probe(args.copy_probe_mb)
+    # force pass: read the FP32 snapshot (16 B) and F'(rho) (4 B), write the
+    # FP32 force (12 B) per atom
+    achieved, _ = kernel_rate(times, "force", 32 * n)
+    force_err = None
+    if not args.no_force_check:
+        from oracle import oracle as O
+        xf = wrap(eng.download(om.X), lo, L, 7)
+        eng.compute()
+        fg = eng.download(om.F)
+        sp = [O.Spline.from_table(tab[k]) for k in ("PAIR", "EDENS", "EMBED")]
+        nb = O.neighbors(xf, lo, L, 7, 5.5 + skin, maxn=96)
+        fo = O.eam_forces(xf, lo, L, 7, nb, *sp)[0]
+        force_err = float(np.sqrt(((fg - fo) ** 2).sum() / (fo ** 2).sum()))
+        log(f"[bench] force relRMS vs FP64 oracle = {force_err:.3e}")
+    cpu = None
+    if args.cpu_steps > 0:
+        from oracle import oracle as O
+        sp = [O.Spline.from_table(tab[k]) for k in ("PAIR", "EDENS", "EMBED")]
+        c0 = time.perf_counter()
+        O.run(1, x, v, m, lo, L, 7, sp, 5.5, 0.3, 10, args.cpu_steps, dt, maxn=96)
+        cpu = {"value": n * args.cpu_steps / (time.perf_counter() - c0),
+               "unit": "atom-steps/s", "cores": 1, "kind": "port",
+               "sample": f"same {n}-atom EAM system, {args.cpu_steps} velocity-Verlet steps "
+                         "of the FP64 C oracle (half list, list rebuilt every 10 steps), "
+                         "one thread"}
+        log(f"[bench] cpu oracle {cpu['value']:.4e} atom-steps/s")
+    return dict(
+        rate=rate, elapsed=elapsed, steps=steps, warmup=warmup, dtype="f32",
+        config={
+            "workload": "EAM Al fcc a=4.05 A, 40^3 cells = 256,000 atoms, PBC, 300 K + "
+                        "0.02 A displacements, Johnson-type EAM 5,000-point splines, "
+                        "rc 5.5 A, dt 2 fs",
+            "atoms": n, "force_dtype": "f32 with compensated per-atom sums",
+            "integrator_dtype": "f64", "force_rel_rms_vs_fp64": force_err,
+            "energy_per_atom_start_end_eV": [e_start, (ek + ep - base) / n],
+            "skin_A": skin,
+            # compulsory bytes per atom-step: integrator 128, density 20, force 32
+            "step_roofline_frac": 180 * rate / (HBM_PEAK_GBS * 1e9)},
+        roofline={"kernel": "k_eam_force", "achieved": achieved,
+                  "traffic": load_traffic("traffic_eam256k.json") if cells == 40 else None},
+        times=times, cpu=cpu)
+
+
+def bench_lj(args):
+    dt, seed = 0.001, 12345
+    cells = args.cells or 10
+    pos, (lo, hi) = lattice.fcc100(cells, cells, cells, 5.26)
+    m = tables.mass_sim(tables.MASS_AR)
+    v = lattice.maxwell_boltzmann(len(pos), m, 40.0, seed)
+    n = len(pos)
+    L = hi - lo
+    skin = 0.3 if args.skin is None else args.skin
+    steps = 1000 if args.steps is None else args.steps
+    warmup = 20 if args.warmup is None else args.warmup
+    tab = tables.lj_table()
+    log(f"[bench] LJ Ar FP64 n={n} steps={steps}")
+    eng = om.Engine()
+    eng.set_box(lo, hi, 7)
+    eng.upload_atoms(pos, v, mass=[m])
+    eng.add_table(om.TAB_PAIR, 0, 0, tab.y, tab.dr, tab.r0, tab.rmult)
+    eng.set_potential(om.POT_PAIR, fp64=True)
+    eng.set_neighbor(skin, args.check_every or 10, 0)
+    base, _ = eng.compute()
+    eng.zero_forces()
+    if warmup:
+        eng.step(warmup, dt)
+    (ek, ep), elapsed, times = timed_steps(eng, lambda: eng.step(steps, dt))
+    rate = n * steps / elapsed
+    log(f"[bench] {steps} steps in {elapsed:.4f} s: {rate:.4e} atom-steps/s")
+    # pair pass: FP64 positions (24 B) read, FP64 force (24 B) written per atom
+    achieved, _ = kernel_rate(times, "force", 48 * n)
+    force_err = None
+    if not args.no_force_check:
+        from oracle import oracle as O
+        xf = wrap(eng.download(om.X), lo, L, 7)
+        eng.compute()
+        fg = eng.download(om.F)
+        nb = O.neighbors(xf, lo, L, 7, 8.5 + skin, maxn=128)
+        fo = O.pair_forces(xf, lo, L, 7, nb, O.Spline.from_table(tab))[0]
+        force_err = float(np.sqrt(((fg - fo) ** 2).sum() / (fo ** 2).sum()))
+        log(f"[bench] force relRMS vs FP64 oracle = {force_err:.3e}")
+    cpu = None
+    if args.cpu_steps > 0:
+        from oracle import oracle as O
+        c0 = time.perf_counter()
+        O.run(0, pos, v, m, lo, L, 7, [O.Spline.from_table(tab)], 8.5, skin, 10,
+              args.cpu_steps * 10, dt, maxn=128)
+        cpu = {"value": n * args.cpu_steps * 10 / (time.perf_counter() - c0),
+               "unit": "atom-steps/s", "cores": 1, "kind": "port",
+               "sample": f"same {n}-atom LJ system, {args.cpu_steps * 10} steps of the FP64 "
+                         "C oracle, one thread"}
+    return dict(
+        rate=rate, elapsed=elapsed, steps=steps, warmup=warmup, dtype="f64",
+        config={"workload": "LJ Ar fcc a=5.26 A, 10^3 cells = 4,000 atoms, PBC, FP64 "
+                            "plain method, eps 0.0104 eV, sigma 3.40 A, rc 2.5 sigma, "
+                            "40 K, dt 1 fs",
+                "atoms": n, "force_dtype": "f64", "integrator_dtype": "f64",
+                "force_rel_rms_vs_fp64": force_err, "skin_A": skin},
+        roofline={"kernel": "k_pair64", "achieved": achieved, "traffic": None},
+        times=times, cpu=cpu)
+
+
+def bench_tersoff(args):
+    dt, seed = 0.001, 4242
+    cells = args.cells or 50
+    pos, (lo, hi) = lattice.diamond100(cells, cells, cells, 5.431)
+    m = tables.mass_sim(tables.MASS_SI)
+    rng = np.random.default_rng(seed)
+    L = hi - lo
+    x = wrap(pos + rng.normal(0.0, 0.05, pos.shape), lo, L, 3)
+    v = lattice.maxwell_boltzmann(len(x), m, 1000.0, seed + 1)
+    n = len(x)
+    skin = 0.5 if args.skin is None else args.skin
+    steps = 1000 if args.steps is None else args.steps
+    warmup = 20 if args.warmup is None else args.warmup
+    tp = params.tersoff_params(os.path.join(REF, "tersoff.silicon"), 2.7, 0.15)
+    log(f"[bench] Tersoff Si n={n} steps={steps}")
+    eng = om.Engine()
+    eng.set_box(lo, hi, 3)
+    eng.upload_atoms(x, v, mass=[m])
+    eng.set_tersoff(tp)
+    eng.set_potential(om.POT_TERSOFF)
+    eng.set_neighbor(skin, args.check_every or 10, 0)
+    base, _ = eng.compute()
+    eng.zero_forces()
+    if warmup:
+        eng.step(warmup, dt)
+    (ek, ep), elapsed, times = timed_steps(eng, lambda: eng.step(steps, dt))
+    rate = n * steps / elapsed
+    log(f"[bench] {steps} steps in {elapsed:.4f} s: {rate:.4e} atom-steps/s")
+    # force pass: snapshot 16 B read, force 12 B written per atom
+    achieved, _ = kernel_rate(times, "force", 28 * n)
+    force_err = None
+    if not args.no_force_check:
+        from oracle import oracle as O
+        xf = wrap(eng.download(om.X), lo, L, 3)
+        eng.compute()
+        fg = eng.download(om.F)
+        nb = O.neighbors(xf, lo, L, 3, 2.85 + skin, maxn=64)
+        fo = O.tersoff_forces(xf, lo, L, 3, nb, O.TersoffParam(*tp, 3))[0]
+        force_err = float(np.sqrt(((fg - fo) ** 2).sum() / (fo ** 2).sum()))
+        log(f"[bench] force relRMS vs FP64 oracle = {force_err:.3e}")
+    cpu = None
+    if args.cpu_steps > 0:
+        from oracle import oracle as O
+        # force evaluations of the oracle on the same system (the oracle has
+        # no Tersoff run driver; the integrator is a small share)
+        nb = O.neighbors(x, lo, L, 3, 2.85, maxn=64)
+        c0 = time.perf_counter()
+        for _ in range(args.cpu_steps):
+            O.tersoff_forces(x, lo, L, 3, nb, O.TersoffParam(*tp, 3))
+        cpu = {"value": n * args.cpu_steps / (time.perf_counter() - c0),
+               "unit": "atom-steps/s (force evaluations)", "cores": 1, "kind": "port",
+               "sample": f"same {n}-atom Tersoff system, {args.cpu_steps} force evaluations "
+                         "of the FP64 C oracle, one thread"}
+    return dict(
+        rate=rate, elapsed=elapsed, steps=steps, warmup=warmup, dtype="f32",
+        config={"workload": "Tersoff Si(B) R=2.7 D=0.15, diamond a=5.431 A, 50^3 cells = "
+                            "1,000,000 atoms, PBC xy, free (001) surfaces, 1000 K, dt 1 fs",
+                "atoms": n, "force_dtype": "f32 with compensated per-atom sums",
+                "integrator_dtype": "f64", "force_rel_rms_vs_fp64": force_err,
+                "skin_A": skin},
+        roofline={"kernel": "k_tersoff_force", "achieved": achieved, "traffic": None},
+        times=times, cpu=cpu)
+
+
+def bench_impact(args):
+    nrep = args.replicas
+    skin = 1.0 if args.skin is None else args.skin
+    steps = 2000 if args.steps is None else args.steps
+    warmup = 20 if args.warmup is None else args.warmup
+    t0 = time.perf_counter()
+    b = impact.batch(nrep, 2024)
+    n = len(b["x"])
+    log(f"[bench] impacts: {nrep} replicas, {n} atoms, set-up {time.perf_counter() - t0:.1f} s")
+    eng = om.Engine()
+    impact.setup_engine(eng, b, os.path.join(REF, "sw.silicon"),
+                        os.path.join(REF, "pair.ar-si.zbl"), skin=skin,
+                        graph_steps=args.check_every or 20)
+    eng.batch_run(1)                    # FirstRun, first list, graph capture
+    if warmup:
+        eng.batch_run(warmup)
+    (t, ek, ep), elapsed, times = timed_steps(eng, lambda: eng.batch_run(steps))
+    rate = n * steps / elapsed
+    log(f"[bench] {steps} steps in {elapsed:.4f} s: {rate:.4e} atom-steps/s; "
+        f"simulated time {1e3 * t.min():.1f}-{1e3 * t.max():.1f} fs")
+    # SW three-body pass: per atom the near list (4 neighbours in the
+    # diamond slab, 16 B each) and its length read, the force read and
+    # written (12 B each)
+    achieved, _ = kernel_rate(times, "force", (4 * 16 + 4 + 24) * n)
+    force_err = None
+    if not args.no_force_check:
+        from oracle import oracle as O
+        eng.compute()
+        xg = eng.download(om.X)
+        fg = eng.download(om.F)
+        y, dr, r0, rm = O.table_load(os.path.join(REF, "pair.ar-si.zbl"), "")
+        p = impact.sw_params(os.path.join(REF, "sw.silicon"))
+        swp, pair = O.SwParam(*p[:10], int(p[10]), *p[11:]), O.Spline(y, dr, r0, rm)
+        num = den = 0.0
+        for r in range(min(nrep, 8)):
+            i0, i1 = b["off"][r], b["off"][r + 1]
+            fo, _ = O.impact_forces(xg[i0:i1], b["tid"][i0:i1], None, b["lo"][r],
+                                    b["hi"][r] - b["lo"][r], 3, swp, pair)
+            num += ((fg[i0:i1] - fo) ** 2).sum()
+            den += (fo ** 2).sum()
+        force_err = float(np.sqrt(num / den))
+        log(f"[bench] force relRMS vs FP64 oracle (first 8 replicas) = {force_err:.3e}")
+    cpu = None
+    if args.cpu_steps > 0:
+        from oracle import oracle as O
+        y, dr, r0, rm = O.table_load(os.path.join(REF, "pair.ar-si.zbl"), "")
+        p = impact.sw_params(os.path.join(REF, "sw.silicon"))
+        swp, pair = O.SwParam(*p[:10], int(p[10]), *p[11:]), O.Spline(y, dr, r0, rm)
+        r = int(np.argmin(np.abs(np.diff(b["off"]) - np.diff(b["off"]).mean())))
+        i0, i1 = b["off"][r], b["off"][r + 1]
+        c0 = time.perf_counter()
+        O.impact_run(b["x"][i0:i1], b["v"][i0:i1], b["tid"][i0:i1], b["flag"][i0:i1],
+                     b["mass"], b["lo"][r], b["hi"][r], 3, swp, pair, args.cpu_steps,
+                     impact.DT_MAX, impact.MAXPATH, impact.UPDATE, 7, impact.DAMP_FACTOR,
+                     impact.RANGE_BORDER)
+        cpu = {"value": (i1 - i0) * args.cpu_steps / (time.perf_counter() - c0),
+               "unit": "atom-steps/s", "cores": 1, "kind": "port",
+               "sample": f"one replica of mean size ({i1 - i0} atoms), {args.cpu_steps} "
+                         "steps of the FP64 oracle impact driver (list rebuilt every step), "
+                         "one thread"}
+    return dict(
+        rate=rate, elapsed=elapsed, steps=steps, warmup=warmup, dtype="f32",
+        config={"workload": f"{nrep} Si(001) slab replicas, 2k-20k atoms log-uniform "
+                            f"({n} atoms), 300 K, Ar 50-500 eV at 0-60 deg, SW+ZBL and "
+                            "Ar-Si ZBL table, DynamicTimeStep maxpath 0.13 A dt<=1 fs, "
+                            "HIP graphs",
+                "atoms": n, "replicas": nrep, "force_dtype": "f32 with compensated sums",
+                "integrator_dtype": "f64", "force_rel_rms_vs_fp64": force_err,
+                "skin_A": skin, "simulated_fs_min_max": [1e3 * t.min(), 1e3 * t.max()]},
+        roofline={"kernel": "k_sw_three", "achieved": achieved, "traffic": None},
+        times=times, cpu=cpu)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--cells", type=int, default=40)
-    ap.add_argument("--skin", type=float, default=0.6)
-    ap.add_argument("--check-every", type=int, default=10)
+    ap.add_argument("--workload", choices=("eam", "lj", "tersoff", "impact"), default="eam")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--cells", type=int, default=0)
+    ap.add_argument("--replicas", type=int, default=512)
+    ap.add_argument("--skin", type=float, default=None)
+    ap.add_argument("--check-every", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=20,
                     help="steps of the FP64 oracle timed on the host (0: skip)")
     ap.add_argument("--no-force-check", action="store_true")
@@ -93,118 +393,31 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus above 1 is not supported "
-              "(independent replicas and domain decomposition are described in DESIGN.md)",
+              "(how replicas and domains would shard is described in DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
-
-    dt = 0.002
-    seed = 12346
-    x, v, lo, hi, m = build_system(args.cells, seed, 0.02, 300.0)
-    n = len(x)
-    model = tables.JohnsonEAM()
-    tab = model.tables(5000)
-    log(f"[bench] EAM Al n={n} skin={args.skin} steps={args.steps} warmup={args.warmup}")
-
-    eng = om.Engine()
-    eng.set_box(lo, hi, 7)
-    eng.upload_atoms(x, v, mass=[m])
-    eng.add_table(om.TAB_PAIR, 0, 0, tab["PAIR"].y, tab["PAIR"].dr, tab["PAIR"].r0)
-    eng.add_table(om.TAB_EDENS, 0, 0, tab["EDENS"].y, tab["EDENS"].dr, tab["EDENS"].r0)
-    eng.add_table(om.TAB_EMBED, 0, 0, tab["EMBED"].y, tab["EMBED"].dr, tab["EMBED"].r0)
-    eng.set_potential(om.POT_EAM)
-    eng.set_neighbor(args.skin, args.check_every, 0)
-    base, _ = eng.compute()            # FirstRun: BasePotential, then f = 0
-    eng.zero_forces()
-    ek0 = 0.5 * m * float((v * v).sum())
-    e_start = ek0 / n                   # Ep - BasePotential = 0 at t = 0
-    if args.warmup:
-        eng.step(args.warmup, dt)
-    log("[bench] warmup done")
-
-    eng.timing_start()
-    t0 = time.perf_counter()
-    ek, ep = eng.step(args.steps, dt)   # returns after the stream drained
-    t1 = time.perf_counter()
-    times = eng.timing_stop()
-    elapsed = t1 - t0
-    rate = n * args.steps / elapsed
-    log(f"[bench] {args.steps} steps in {elapsed:.4f} s: {rate:.4e} atom-steps/s")
-    e_end = (ek + ep - base) / n
-    if args.copy_probe_mb:
-        copy_probe(args.copy_probe_mb)
-
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_eam256k.json")
-    if os.path.exists(tpath) and args.cells == 40:
-        with open(tpath) as fl:
-            traffic = json.load(fl).get("traffic_bytes_per_launch")
-    force_ms, force_launches = times["force"]
-    avg_force_s = force_ms / max(force_launches, 1) / 1e3
-    achieved = FORCE_PASS_BYTES * n / avg_force_s / 1e9 if avg_force_s > 0 else 0.0
-
-    # force error of the final state against the FP64 oracle
-    force_err = None
-    if not args.no_force_check:
-        from oracle import oracle as O
-        xf = eng.download(om.X)
-        eng.compute()
-        fg = eng.download(om.F)
-        sp = [O.Spline.from_table(tab[k]) for k in ("PAIR", "EDENS", "EMBED")]
-        nb = O.neighbors(xf, lo, hi - lo, 7, 5.5 + args.skin, maxn=96)
-        fo = O.eam_forces(xf, lo, hi - lo, 7, nb, *sp)[0]
-        force_err = float(np.sqrt(((fg - fo) ** 2).sum() / (fo ** 2).sum()))
-        log(f"[bench] force relRMS vs FP64 oracle = {force_err:.3e}")
-
-    cpu = None
-    if args.cpu_steps > 0:
-        from oracle import oracle as O
-        sp = [O.Spline.from_table(tab[k]) for k in ("PAIR", "EDENS", "EMBED")]
-        c0 = time.perf_counter()
-        O.run(1, x, v, m, lo, hi - lo, 7, sp, 5.5, 0.3, 10, args.cpu_steps, dt, maxn=96)
-        c1 = time.perf_counter()
-        cpu = {"value": n * args.cpu_steps / (c1 - c0), "unit": "atom-steps/s", "cores": 1,
-               "kind": "port",
-               "sample": f"same {n}-atom EAM system, {args.cpu_steps} velocity-Verlet steps "
-                         "of the FP64 C oracle (half list, list rebuilt every 10 steps), "
-                         "one thread"}
-        log(f"[bench] cpu oracle {cpu['value']:.4e} atom-steps/s")
-
+    r = {"eam": bench_eam, "lj": bench_lj, "tersoff": bench_tersoff,
+         "impact": bench_impact}[args.workload](args)
+    roof = r["roofline"]
     out = {
-        "metric": "atom-steps/sec at stated force error vs FP64 oracle; % HBM roofline",
-        "value": rate,
+        "metric": METRIC,
+        "value": r["rate"],
         "unit": "atom-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "steps": r["steps"],
+        "warmup": r["warmup"],
+        "ms_per_step": r["elapsed"] / r["steps"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": r["dtype"],
         "data": "synthetic",
-        "config": {
-            "workload": "EAM Al fcc a=4.05 A, 40^3 cells = 256,000 atoms, PBC, 300 K + "
-                        "0.02 A displacements, Johnson-type EAM 5,000-point splines, "
-                        "rc 5.5 A, dt 2 fs",
-            "atoms": n,
-            "force_dtype": "f32 with compensated per-atom sums",
-            "integrator_dtype": "f64",
-            "force_rel_rms_vs_fp64": force_err,
-            "energy_per_atom_start_end_eV": [e_start, e_end],
-            "skin_A": args.skin,
-            "step_roofline_frac": STEP_BYTES * rate / (HBM_PEAK_GBS * 1e9),
-            "kernel_ms": {k: {"total": t, "launches": c} for k, (t, c) in times.items()},
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_eam_force",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-        },
-        "cpu_baseline": cpu,
+        "config": dict(r["config"], kernel_ms={k: {"total": t, "launches": c}
+                                               for k, (t, c) in r["times"].items()}),
+        "roofline": {"bound": "hbm", "kernel": roof["kernel"], "achieved": roof["achieved"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": roof["achieved"] / HBM_PEAK_GBS, "traffic": roof["traffic"]},
+        "cpu_baseline": r["cpu"],
     }
     print(json.dumps(out), flush=True)
 

@@ -110,30 +110,102 @@ __global__ void k_bin(const double* __restrict__ x, int n, GridDev g, int* __res
 }
 
 // exclusive scan of count[0..nc) into start[0..nc], count reset to zero
-__global__ void __launch_bounds__(1024) k_scan(int* __restrict__ count, int* __restrict__ start,
-                                               int nc) {
-  __shared__ int part[1024];
+// Exclusive scan of the cell counts in tiles of kTile cells:
+//   k_scan_tiles  per-tile totals (coalesced loads)
+//   k_scan_top    exclusive scan of the tile totals (one workgroup)
+//   k_scan_apply  the tile through LDS: per-thread runs of kScanPer cells,
+//                 a workgroup scan of the runs, coalesced stores of the
+//                 starts; counts reset to zero for the scatter
+constexpr int kScanT = 256, kScanPer = 16, kTile = kScanT * kScanPer;
+
+__device__ inline int block_excl_scan(int v, int* sh, int& total) {
   const int t = threadIdx.x;
-  const int per = (nc + 1023) / 1024;
-  const int b = t * per, e = min(nc, b + per);
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanT; off <<= 1) {
+    const int w = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += w;
+    __syncthreads();
+  }
+  total = sh[kScanT - 1];
+  const int r = sh[t] - v;
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(kScanT) k_scan_tiles(const int* __restrict__ count, int nc,
+                                                       int* __restrict__ part) {
+  __shared__ int sh[kScanT];
+  const int base = blockIdx.x * kTile;
   int s = 0;
-  for (int c = b; c < e; c++) s += count[c];
-  part[t] = s;
+  for (int k = 0; k < kScanPer; k++) {
+    const int c = base + k * kScanT + (int)threadIdx.x;
+    if (c < nc) s += count[c];
+  }
+  int total;
+  block_excl_scan(s, sh, total);
+  if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_top(int* __restrict__ part, int nb) {
+  __shared__ int sh[1024];
+  const int t = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int b = t * per, e = min(nb, b + per);
+  int s = 0;
+  for (int i = b; i < e; i++) s += part[i];
+  sh[t] = s;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
-    int v = t >= off ? part[t - off] : 0;
+    const int v = t >= off ? sh[t - off] : 0;
     __syncthreads();
-    part[t] += v;
+    sh[t] += v;
     __syncthreads();
   }
-  int run = part[t] - s;
-  for (int c = b; c < e; c++) {
-    int v = count[c];
-    start[c] = run;
+  int run = sh[t] - s;
+  for (int i = b; i < e; i++) {
+    const int v = part[i];
+    part[i] = run;
     run += v;
-    count[c] = 0;
   }
-  if (t == 1023) start[nc] = part[1023];
+}
+
+// LDS index with one pad word per kScanPer words: the per-thread runs of
+// kScanPer consecutive cells then fall in distinct banks
+__device__ inline int pad(int i) { return i + i / kScanPer; }
+
+__global__ void __launch_bounds__(kScanT) k_scan_apply(int* __restrict__ count, int nc,
+                                                       const int* __restrict__ part,
+                                                       int* __restrict__ start, int total_n) {
+  __shared__ int tile[kTile + kTile / kScanPer];
+  __shared__ int sh[kScanT];
+  const int t = threadIdx.x;
+  const int base = blockIdx.x * kTile;
+  for (int k = 0; k < kScanPer; k++) {
+    const int i = k * kScanT + t;
+    tile[pad(i)] = base + i < nc ? count[base + i] : 0;
+  }
+  __syncthreads();
+  int s = 0;
+  for (int k = 0; k < kScanPer; k++) s += tile[pad(t * kScanPer + k)];
+  int total;
+  int run = part[blockIdx.x] + block_excl_scan(s, sh, total);
+  for (int k = 0; k < kScanPer; k++) {
+    const int i = pad(t * kScanPer + k);
+    const int v = tile[i];
+    tile[i] = run;
+    run += v;
+  }
+  __syncthreads();
+  for (int k = 0; k < kScanPer; k++) {
+    const int i = k * kScanT + t;
+    if (base + i < nc) {
+      start[base + i] = tile[pad(i)];
+      count[base + i] = 0;
+    }
+  }
+  if (blockIdx.x == 0 && t == 0) start[nc] = total_n;
 }
 
 __global__ void k_scatter(const int* __restrict__ cellid, int n, const int* __restrict__ start,
@@ -173,6 +245,25 @@ __global__ void __launch_bounds__(256) k_cell_sort(int* __restrict__ perm,
       }
       perm[j + 1] = p;
     }
+  }
+}
+
+// Small cells (a few atoms): one lane per cell, insertion sort by index.
+__global__ void __launch_bounds__(256) k_cell_sort_lane(int* __restrict__ perm,
+                                                        const int* __restrict__ start, int nc,
+                                                        const int* __restrict__ id) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const int b = start[c], e = start[c + 1];
+  for (int i = b + 1; i < e; i++) {
+    const int p = perm[i];
+    const int key = id[p];
+    int j = i - 1;
+    while (j >= b && id[perm[j]] > key) {
+      perm[j + 1] = perm[j];
+      j--;
+    }
+    perm[j + 1] = p;
   }
 }
 
@@ -388,6 +479,13 @@ void nbr_build(Ctx& c) {
     OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.cell_fill), sizeof(int) * (nc + 1)));
     c.cap_cells = nc + 1;
   }
+  const int tiles = (int)((nc + kTile - 1) / kTile);
+  if (tiles > c.cap_scan) {
+    (void)hipFree(c.scan_part);
+    c.scan_part = nullptr;
+    OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.scan_part), sizeof(int) * tiles));
+    c.cap_scan = tiles;
+  }
   if ((long)c.maxnbr * n > c.cap_nbr) throw ApiFail(OMDG_EINVAL, "list storage too small");
   GridDev gd;
   for (int d = 0; d < 3; d++) {
@@ -401,11 +499,19 @@ void nbr_build(Ctx& c) {
   OMDG_HIP(hipMemsetAsync(c.cell_fill, 0, sizeof(int) * (nc + 1), c.stream));
   const int src = c.cur, dst = 1 - c.cur;
   k_bin<<<grid_blocks(n, B), B, 0, c.stream>>>(c.x[src], n, gd, c.cellid, c.cell_fill);
-  k_scan<<<1, 1024, 0, c.stream>>>(c.cell_fill, c.cell_start, (int)nc);
+  k_scan_tiles<<<tiles, kScanT, 0, c.stream>>>(c.cell_fill, (int)nc, c.scan_part);
+  k_scan_top<<<1, 1024, 0, c.stream>>>(c.scan_part, tiles);
+  k_scan_apply<<<tiles, kScanT, 0, c.stream>>>(c.cell_fill, (int)nc, c.scan_part, c.cell_start,
+                                               n);
   k_scatter<<<grid_blocks(n, B), B, 0, c.stream>>>(c.cellid, n, c.cell_start, c.cell_fill,
                                                    c.perm);
-  k_cell_sort<<<grid_blocks(nc * 64, B), B, 0, c.stream>>>(c.perm, c.cell_start, (int)nc,
-                                                            c.id[src]);
+  // a wavefront per cell when cells hold several atoms, a lane otherwise
+  if ((long)n > 4 * nc)
+    k_cell_sort<<<grid_blocks(nc * 64, B), B, 0, c.stream>>>(c.perm, c.cell_start, (int)nc,
+                                                              c.id[src]);
+  else
+    k_cell_sort_lane<<<grid_blocks(nc, B), B, 0, c.stream>>>(c.perm, c.cell_start, (int)nc,
+                                                              c.id[src]);
   int* cell_dst = c.cellid2;
   k_permute<<<grid_blocks(n, B), B, 0, c.stream>>>(
       n, c.perm, c.x[src], c.v[src], c.type[src], c.flag[src], c.id[src], c.cellid, c.x[dst],

@@ -256,6 +256,8 @@ static void raise_device_errors(Ctx& c) {
   if (e & ERR_SKIN)
     throw ApiFail(OMDG_ENBR, "an atom moved more than skin/2 between list checks");
   if (e & ERR_WRAP) throw ApiFail(OMDG_ENBR, "an atom left the periodic box by more than one length");
+  if (e & ERR_NEAR)
+    throw ApiFail(OMDG_ENBR, "more than 24 three-body neighbours inside the cut of one atom");
   throw ApiFail(OMDG_EINVAL, "device error flags " + std::to_string(e));
 }
 
@@ -303,6 +305,19 @@ static void rebuild(Ctx& c) {
     drop_graph(c);
     batch_radii(c);
     batch_setup_grids(c, c.rep_lo.data(), c.rep_hi.data(), c.rep_pbc.data());
+    if (c.maxnbr_req == 0) {
+      // lists are rebuilt inside captured steps, where they cannot grow:
+      // room for an atom embedded at 1.5x the densest replica's mean density
+      double dens = 0.0, rl2 = 0.0;
+      for (int r = 0; r < c.nrep; r++) {
+        const RepGrid& g = c.rep_grid_h[r];
+        const double vol = g.M[0] * g.h[0] * g.M[1] * g.h[1] * g.M[2] * g.h[2];
+        dens = std::max(dens, (c.rep_off[r + 1] - c.rep_off[r]) / std::max(vol, 1.0));
+      }
+      for (float v : c.rl2_type) rl2 = std::max(rl2, (double)v);
+      const double est = 1.5 * dens * 4.18879020478639 * rl2 * std::sqrt(rl2);
+      c.maxnbr = std::max(c.maxnbr, ((int)est + 15) / 16 * 16);
+    }
   }
   for (int attempt = 0; attempt < 3; attempt++) {
     size_t need = (size_t)c.maxnbr * (size_t)c.n;
@@ -463,6 +478,8 @@ int omdg_destroy(omdg_t h) {
   dfree(c.nbr);
   dfree(c.bond_b);
   dfree(c.bond_p);
+  dfree(c.near);
+  dfree(c.near_n);
   dfree(c.cell_start);
   dfree(c.cell_fill);
   dfree(c.tab32);
@@ -482,6 +499,7 @@ int omdg_destroy(omdg_t h) {
   dfree(c.rep_v2);
   dfree(c.dstep);
   dfree(c.cell_rep);
+  dfree(c.scan_part);
   dfree(c.rep_need);
   dfree(c.rep_d2);
   dfree(c.rep_vcur);

@@ -33,6 +33,7 @@ enum : int {
   ERR_NBR_OVERFLOW = 8,
   ERR_SKIN = 16,
   ERR_WRAP = 32,
+  ERR_NEAR = 64,     // more three-body neighbours inside the cut than kept
 };
 
 // One spline table on the device.  Coefficients {a,b,c,d} per knot.
@@ -125,6 +126,9 @@ struct Ctx {
   float* bond_b = nullptr;  // Tersoff per-slot bond order (ELL like nbr)
   float* bond_p = nullptr;  // Tersoff per-slot zeta prefactor
   size_t cap_bond = 0;
+  float4* near = nullptr;   // SW near lists (displacement, entry) per slot
+  int* near_n = nullptr;
+  size_t cap_near = 0;
   int cap_atoms = 0;
   // neighbour list (ELL, column-major: nbr[k * n + i])
   uint32_t* nbr = nullptr;
@@ -186,6 +190,8 @@ struct Ctx {
   long rep_steps = 0;          // steps run since omdg_set_replicas
   int* dstep = nullptr;        // device step counter (DynamicTimeStep cadence)
   long rep_ncell = 0;
+  int* scan_part = nullptr;    // device: tile totals of the cell-count scan
+  int cap_scan = 0;
   int* cell_rep = nullptr;     // device: replica of each cell
   int* rep_need = nullptr;     // device: rebuild replica r in this step
   unsigned int* rep_d2 = nullptr;  // device: max displacement^2 (float bits)

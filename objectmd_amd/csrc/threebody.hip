@@ -2,16 +2,18 @@
 // accumulates its own force; no atomics, deterministic).  FP32 arithmetic on
 // the cell-relative snapshots with Kahan-compensated per-atom sums.
 //
-// Stillinger-Weber (src/gadget/sw.cpp):
-//   two-body  ZBL core below sx0, polynomial splice below sx1, SW beyond
-//             (sw.cpp:118-189, 240-252); pairs of other type combinations
-//             with a PAIR table use the table (TForcePair, tpair.h:62-71)
-//   three-body h(j,i,k) for every centre i and unordered pair {j,k} of its
-//             neighbours inside a*sigma (sw.cpp:191-236, 254-273).  Atom l
-//             gathers the triplets where it is the centre and those where it
-//             is a leg (centre c = a neighbour of l, other leg k from c's
-//             list); each triplet gives a third of its energy to each atom,
-//             as the reference does.
+// Stillinger-Weber (src/gadget/sw.cpp), two passes:
+//   pass 1    two-body terms: ZBL core below sx0, polynomial splice below
+//             sx1, SW beyond (sw.cpp:118-189, 240-252); pairs of other type
+//             combinations with a PAIR table use the table (TForcePair,
+//             tpair.h:62-71).  Writes each atom's near list: its SW
+//             neighbours inside a*sigma with their displacement vectors.
+//   pass 2    three-body h(j,i,k) for every centre i and unordered pair
+//             {j,k} of its near neighbours (sw.cpp:191-236, 254-273).  Atom
+//             l gathers the triplets where it is the centre and those where
+//             it is a leg (centre c = a near neighbour of l, other leg k
+//             from c's near list); each triplet gives a third of its energy
+//             to each atom, as the reference does.
 // Tersoff (PRB 37 6991 with the cutoff of include/potential/tersoff.h:247-257)
 //   pass 1    per bond (i,j): zeta_ij, b_ij, the pair energy 1/2 V_ij and
 //             P_ij = 1/2 fc fA db/dzeta stored per list slot
@@ -28,7 +30,7 @@ namespace omdg {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxNear = 32;  // neighbours inside the cut kept per atom
+constexpr int kMaxNear = 24;  // three-body neighbours inside the cut kept per atom
 
 struct TBArgs {
   const float4* q;
@@ -39,6 +41,8 @@ struct TBArgs {
   float* vir;
   float* bond_b;  // Tersoff: b_ij per list slot
   float* bond_p;  // Tersoff: P_ij per list slot
+  float4* near;   // SW: near list, kMaxNear rows of n (x_j - x_l, entry bits)
+  int* near_n;    // SW: near-list length per atom
   int* err;
   int n;
   float hx, hy, hz;
@@ -57,6 +61,7 @@ struct TBArgs {
   TabView tabs[kMaxTypes * kMaxTypes];
 };
 
+// a neighbour inside the cut (Tersoff passes)
 struct Near {
   float dx, dy, dz, r;
   uint32_t e;
@@ -132,22 +137,23 @@ __device__ inline float4 off_of(uint32_t code, const float4& h) {
                      (float)(c / 9 - 1) * h.z, 0.f);
 }
 
+// Pass 1: two-body terms (SW with its ZBL core and splice, pair tables for
+// the other type pairs) and the compact near list of every atom: the SW
+// neighbours inside a*sigma as {x_j - x_l, entry} in slot-major float4 rows.
 template <bool ENERGY>
-__global__ void __launch_bounds__(kBlock) k_sw(TBArgs a) {
+__global__ void __launch_bounds__(kBlock) k_sw_pair(TBArgs a) {
   const int l = xcd_block() * blockDim.x + threadIdx.x;
   if (l >= a.n) return;
   const int n = a.n;
-  // neighbours and their centres belong to the same replica
+  // neighbours belong to the atom's replica
   const float4 hh = a.rep ? a.rep_h[a.rep[l]] : make_float4(a.hx, a.hy, a.hz, 0.f);
   const float4 ql = a.q[l];
   const int tl = snap_type(ql.w);
   const bool l_sw = a.sw_type < 0 || tl == a.sw_type;
   Kahan fx, fy, fz;
   float pot = 0.f, vir = 0.f;
-  Near nr[kMaxNear];
   int nn = 0;
   const int cnt = a.nnbr[l];
-  // two-body terms and the neighbours inside the SW cut
   for (int k = 0; k < cnt; k++) {
     const uint32_t e = a.nbr[nidx(l, k, n)];
     const float4 qj = a.q[e & kSlotMask];
@@ -169,8 +175,8 @@ __global__ void __launch_bounds__(kBlock) k_sw(TBArgs a) {
         fy.add(-fr * d.y);
         fz.add(-fr * d.z);
         if (ENERGY) pot += eh;
-        if (nn < kMaxNear) nr[nn] = {d.x, d.y, d.z, r, e};
-        else atomicOr(a.err, ERR_NBR_OVERFLOW);
+        if (nn < kMaxNear) a.near[(size_t)nn * n + l] = make_float4(d.x, d.y, d.z, __uint_as_float(e));
+        else atomicOr(a.err, ERR_NEAR);
         nn++;
       }
     } else {
@@ -191,45 +197,62 @@ __global__ void __launch_bounds__(kBlock) k_sw(TBArgs a) {
       }
     }
   }
-  if (nn > kMaxNear) nn = kMaxNear;
+  a.near_n[l] = nn < kMaxNear ? nn : kMaxNear;
+  a.f[l] = fx.s;
+  a.f[(size_t)n + l] = fy.s;
+  a.f[2 * (size_t)n + l] = fz.s;
+  if (ENERGY) {
+    a.pot[l] = pot;
+    a.vir[l] = vir;
+  }
+}
+
+// Pass 2: three-body terms from the near lists.  Atom l gathers the
+// triplets centred at l and those centred at a near neighbour c with l as
+// one leg (the other leg k from c's near list); each gives a third of its
+// energy to l.  The sums continue from the pass-1 forces.
+template <bool ENERGY>
+__global__ void __launch_bounds__(kBlock) k_sw_three(TBArgs a) {
+  const int l = xcd_block() * blockDim.x + threadIdx.x;
+  if (l >= a.n) return;
+  const int n = a.n;
+  const int nn = a.near_n[l];
+  if (nn == 0) return;
+  Kahan fx, fy, fz;
+  fx.s = a.f[l];
+  fy.s = a.f[(size_t)n + l];
+  fz.s = a.f[2 * (size_t)n + l];
+  float pot = 0.f;
   const float third = 1.0f / 3.0f;
-  // triplets centred at l
-  for (int u = 0; u < nn; u++)
+  for (int u = 0; u < nn; u++) {
+    const float4 nu = a.near[(size_t)u * n + l];
+    const float d1[3] = {nu.x, nu.y, nu.z};
+    const float r1 = sqrtf(nu.x * nu.x + nu.y * nu.y + nu.z * nu.z);
+    // triplets centred at l: legs u < w
     for (int w = u + 1; w < nn; w++) {
-      const float d1[3] = {nr[u].dx, nr[u].dy, nr[u].dz};
-      const float d2[3] = {nr[w].dx, nr[w].dy, nr[w].dz};
+      const float4 nw = a.near[(size_t)w * n + l];
+      const float d2[3] = {nw.x, nw.y, nw.z};
       SwTerm t;
-      sw_three(a, d1, nr[u].r, d2, nr[w].r, t);
+      sw_three(a, d1, r1, d2, sqrtf(nw.x * nw.x + nw.y * nw.y + nw.z * nw.z), t);
       fx.add(-(t.gj[0] + t.gk[0]));
       fy.add(-(t.gj[1] + t.gk[1]));
       fz.add(-(t.gj[2] + t.gk[2]));
       if (ENERGY) pot += third * t.e;
     }
-  // triplets centred at a neighbour c with l as one leg
-  for (int u = 0; u < nn; u++) {
-    const uint32_t ec = nr[u].e;
-    const int c = (int)(ec & kSlotMask);
-    const float4 qc = a.q[c];
-    // leg l seen from c: d1 = x_l - x_c
-    const float d1[3] = {-nr[u].dx, -nr[u].dy, -nr[u].dz};
-    const float r1 = nr[u].r;
-    const int cc = a.nnbr[c];
-    for (int k = 0; k < cc; k++) {
-      const uint32_t e = a.nbr[nidx(c, k, n)];
-      const int t = (int)(e & kSlotMask);
-      const float4 qk = a.q[t];
-      if (a.sw_type >= 0 && snap_type(qk.w) != a.sw_type) continue;
-      const float3 d = disp(qc, qk, off_of(e >> kImgShift, hh));  // x_k - x_c
-      const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
-      if (!(r2 < a.rc2)) continue;
-      if (t == l) {
-        const float ex = d.x - d1[0], ey = d.y - d1[1], ez = d.z - d1[2];
+    // triplets centred at c = u's atom; leg l seen from c: x_l - x_c
+    const int c = (int)(__float_as_uint(nu.w) & kSlotMask);
+    const float e1[3] = {-nu.x, -nu.y, -nu.z};
+    const int nc = a.near_n[c];
+    for (int w = 0; w < nc; w++) {
+      const float4 nk = a.near[(size_t)w * n + c];  // x_k - x_c
+      if ((int)(__float_as_uint(nk.w) & kSlotMask) == l) {
+        const float ex = nk.x - e1[0], ey = nk.y - e1[1], ez = nk.z - e1[2];
         if (ex * ex + ey * ey + ez * ez < 1e-4f) continue;  // the same image of l
       }
-      const float d2[3] = {d.x, d.y, d.z};
+      const float d2[3] = {nk.x, nk.y, nk.z};
       SwTerm tt;
-      sw_three(a, d1, r1, d2, sqrtf(r2), tt);
-      // each unordered triplet appears once per leg: l as leg j
+      sw_three(a, e1, r1, d2, sqrtf(nk.x * nk.x + nk.y * nk.y + nk.z * nk.z), tt);
+      // l as leg j of the triplet (c; l, k)
       fx.add(tt.gj[0]);
       fy.add(tt.gj[1]);
       fz.add(tt.gj[2]);
@@ -239,10 +262,7 @@ __global__ void __launch_bounds__(kBlock) k_sw(TBArgs a) {
   a.f[l] = fx.s;
   a.f[(size_t)n + l] = fy.s;
   a.f[2 * (size_t)n + l] = fz.s;
-  if (ENERGY) {
-    a.pot[l] = pot;
-    a.vir[l] = vir;
-  }
+  if (ENERGY) a.pot[l] += pot;
 }
 
 // ----------------------------------------------------------- Tersoff ------
@@ -313,7 +333,7 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_bond(TBArgs a) {
         nr[nn] = {d.x, d.y, d.z, sqrtf(r2), e};
         slot[nn] = k;
       } else {
-        atomicOr(a.err, ERR_NBR_OVERFLOW);
+        atomicOr(a.err, ERR_NEAR);
       }
       nn++;
     } else {
@@ -557,10 +577,29 @@ void sw_forces(Ctx& c, bool energy) {
       a.tcut2[s] = (float)(cut * cut);
     }
   a.sw_type = any_tab ? 0 : -1;
+  if ((size_t)c.n * kMaxNear > c.cap_near) {
+    if (c.near) (void)hipFree(c.near);
+    if (c.near_n) (void)hipFree(c.near_n);
+    c.near = nullptr;
+    c.near_n = nullptr;
+    c.cap_near = 0;
+    OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.near), sizeof(float4) * kMaxNear * (size_t)c.n));
+    OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.near_n), sizeof(int) * (size_t)c.n));
+    c.cap_near = (size_t)c.n * kMaxNear;
+  }
+  a.near = c.near;
+  a.near_n = c.near_n;
   const int nb = grid_blocks(c.n, kBlock);
-  TimerScope ts(c, TG_FORCE);
-  if (energy) k_sw<true><<<nb, kBlock, 0, c.stream>>>(a);
-  else k_sw<false><<<nb, kBlock, 0, c.stream>>>(a);
+  {
+    TimerScope ts(c, TG_PASS1);
+    if (energy) k_sw_pair<true><<<nb, kBlock, 0, c.stream>>>(a);
+    else k_sw_pair<false><<<nb, kBlock, 0, c.stream>>>(a);
+  }
+  {
+    TimerScope ts(c, TG_FORCE);
+    if (energy) k_sw_three<true><<<nb, kBlock, 0, c.stream>>>(a);
+    else k_sw_three<false><<<nb, kBlock, 0, c.stream>>>(a);
+  }
   OMDG_HIP(hipGetLastError());
 }
 

@@ -19,6 +19,7 @@ import os
 import numpy as np
 
 from . import FLAG_ACTIVE, FLAG_DAMP, FLAG_FIXED, FLAG_LIMIT, lattice, tables
+from .params import sw_params
 
 A_SI = 5.431
 MAXPATH = 0.13          # omd-parameter:27
@@ -27,31 +28,6 @@ UPDATE = 5              # omd-parameter:29
 DAMP_FACTOR = 0.34328   # omd-parameter:24
 RANGE_BORDER = 1000.0   # main.cpp:76
 PROJ_HEIGHT = 5.0       # omd-parameter:3
-
-
-def read_params(path):
-    """key value... lines of an Object-MD parameter file (tables/sw.silicon)."""
-    out = {}
-    with open(path) as fl:
-        for line in fl:
-            t = line.split()
-            if not t or t[0].startswith("#"):
-                continue
-            out[t[0]] = t[1:]
-    return out
-
-
-def sw_params(path, z1=14.0, z2=14.0):
-    """The 19 values of omdg_set_sw from a tables/sw.<material> file
-    (keys of sw.cpp:35-60); z1, z2 are the atomic numbers of the ZBL core."""
-    p = read_params(path)
-    g = lambda k, i=0: float(p[k][i])
-    usezbl = p.get("zbl", ["no"])[0] in ("yes", "true", "1")
-    return [g("sw.epsilon"), g("sw.A"), g("sw.B"), g("sw.p"), g("sw.q"), g("sw.sigma"),
-            g("sw.a"), g("sw.lambda"), g("sw.gamma"), g("sw.invers_cos0"),
-            1.0 if usezbl else 0.0, g("spline.range", 0), g("spline.range", 1),
-            g("spline.coeff", 0), g("spline.coeff", 1), g("spline.coeff", 2),
-            g("spline.coeff", 3), z1, z2]
 
 
 def projectile(tx, ty, h, theta, phi, energy, mass):

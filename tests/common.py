@@ -5,7 +5,7 @@ import os
 
 import numpy as np
 
-from objectmd_amd import lattice, tables
+from objectmd_amd import lattice, params, tables
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -75,36 +75,12 @@ SW_SILICON = os.path.join(GOLDEN, "ref", "sw.silicon")
 TERSOFF_SILICON = os.path.join(GOLDEN, "ref", "tersoff.silicon")
 
 
-def read_params(path):
-    """key value... lines of a reference parameter file."""
-    out = {}
-    with open(path) as fl:
-        for line in fl:
-            t = line.split()
-            if not t or t[0].startswith("#"):
-                continue
-            out[t[0]] = t[1:]
-    return out
-
-
 def sw_params(z1=14.0, z2=14.0):
     """19 values in the order of omdg_set_sw from tables/sw.silicon."""
-    p = read_params(SW_SILICON)
-    g = lambda k, i=0: float(p[k][i])
-    usezbl = p.get("zbl", ["no"])[0] in ("yes", "true", "1")
-    return [g("sw.epsilon"), g("sw.A"), g("sw.B"), g("sw.p"), g("sw.q"), g("sw.sigma"),
-            g("sw.a"), g("sw.lambda"), g("sw.gamma"), g("sw.invers_cos0"),
-            1.0 if usezbl else 0.0, g("spline.range", 0), g("spline.range", 1),
-            g("spline.coeff", 0), g("spline.coeff", 1), g("spline.coeff", 2),
-            g("spline.coeff", 3), z1, z2]
+    return params.sw_params(SW_SILICON, z1, z2)
 
 
 def tersoff_params(R=None, D=None):
     """13 values in the order of omdg_set_tersoff from tables/tersoff.silicon
     (Si(B), PRB 37 6991); R and D may be overridden."""
-    p = read_params(TERSOFF_SILICON)
-    g = lambda k: float(p[k][0])
-    return [g("tersoff.A"), g("tersoff.B"), g("tersoff.lambda_1"), g("tersoff.lambda_2"),
-            g("tersoff.lambda_3"), g("tersoff.alpha"), g("tersoff.beta"), g("tersoff.n"),
-            g("tersoff.c"), g("tersoff.d"), g("tersoff.h"),
-            g("tersoff.R") if R is None else R, g("tersoff.D") if D is None else D]
+    return params.tersoff_params(TERSOFF_SILICON, R, D)
