@@ -303,7 +303,7 @@ def bench_tersoff(args):
 
 def bench_impact(args):
     nrep = args.replicas
-    skin = 1.0 if args.skin is None else args.skin
+    skin = 0.6 if args.skin is None else args.skin
     steps = 2000 if args.steps is None else args.steps
     warmup = 20 if args.warmup is None else args.warmup
     t0 = time.perf_counter()

@@ -126,11 +126,31 @@ __global__ void k_batch_scatter(const int* __restrict__ cellid, int n,
   perm[start[c] + atomicAdd(fill + c, 1)] = i;
 }
 
-// each cell in ascending atom index, one wavefront per cell (rank sort)
+// Candidate word of the cell-ordered snapshot copy: atom index, type and
+// the active flag (the atom's cell is known from the stencil).
+constexpr int kCandTypeShift = 27;
+constexpr uint32_t kCandActive = 1u << 29;
+static_assert(kCandTypeShift == kImgShift, "candidate index field = list slot field");
+
+__device__ inline float4 cand(const float4* q, const int* flag, int has_flags, int t) {
+  float4 v = q[t];
+  uint32_t w = (uint32_t)t | ((uint32_t)snap_type(v.w) << kCandTypeShift);
+  if (!has_flags || (flag[t] & OMDG_FLAG_ACTIVE)) w |= kCandActive;
+  v.w = __uint_as_float(w);
+  return v;
+}
+
+// Each cell in ascending atom index, one wavefront per cell (rank sort),
+// then the cell's packed candidates {snapshot xyz, index | type | active}
+// in the same order, so the build reads a cell as one contiguous run.
 __global__ void __launch_bounds__(256) k_batch_cell_sort(int* __restrict__ perm,
                                                          const int* __restrict__ start, int nc,
                                                          const int* __restrict__ cell_rep,
-                                                         const int* __restrict__ need) {
+                                                         const int* __restrict__ need,
+                                                         const float4* __restrict__ q,
+                                                         const int* __restrict__ flag,
+                                                         int has_flags,
+                                                         float4* __restrict__ qs) {
   const int c = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const int lane = threadIdx.x % 64;
   if (c >= nc || !need[cell_rep[c]]) return;  // uniform per wavefront
@@ -139,8 +159,13 @@ __global__ void __launch_bounds__(256) k_batch_cell_sort(int* __restrict__ perm,
     const int p = lane < m ? perm[b + lane] : 0x7fffffff;
     int rank = 0;
     for (int u = 0; u < m; u++) rank += (__shfl(p, u, 64) < p) ? 1 : 0;
-    if (lane < m) perm[b + rank] = p;
-  } else if (lane == 0) {
+    if (lane < m) {
+      perm[b + rank] = p;
+      qs[b + rank] = cand(q, flag, has_flags, p);
+    }
+    return;
+  }
+  if (lane == 0) {  // crowded cell: insertion sort by one lane
     for (int i = b + 1; i < e; i++) {
       const int p = perm[i];
       int j = i - 1;
@@ -150,6 +175,7 @@ __global__ void __launch_bounds__(256) k_batch_cell_sort(int* __restrict__ perm,
       }
       perm[j + 1] = p;
     }
+    for (int u = b; u < e; u++) qs[u] = cand(q, flag, has_flags, perm[u]);
   }
 }
 
@@ -167,6 +193,7 @@ struct BBArgs {
   int* nnbr;
   float4* q;
   float4* qb;
+  const float4* qs;  // packed candidates in cell order
   int* stat;
   int n;
   int maxnbr;
@@ -249,11 +276,11 @@ __global__ void __launch_bounds__(kBlock) k_batch_build(BBArgs a) {
         const float ox = (float)dx * hx, oy = (float)dy * hy, oz = (float)dz * hz;
         const int e = a.start[cc + 1];
         for (int u = a.start[cc]; u < e; u++) {
-          const int t = a.perm[u];
-          if (t == s && noshift) continue;
-          if (a.has_flags && !(a.flag[t] & OMDG_FLAG_ACTIVE)) continue;
-          const float4 qt = a.q[t];
-          const int pt = ti * kMaxTypes + snap_type(qt.w);
+          const float4 qt = a.qs[u];
+          const uint32_t w = __float_as_uint(qt.w);
+          const int t = (int)(w & kSlotMask);
+          if ((t == s && noshift) || !(w & kCandActive)) continue;
+          const int pt = ti * kMaxTypes + (int)((w >> kCandTypeShift) & 3u);
           const float fx = qi.x - qt.x - ox, fy = qi.y - qt.y - oy, fz = qi.z - qt.z - oz;
           const float r2f = fx * fx + fy * fy + fz * fz;
           bool in;
@@ -301,7 +328,7 @@ struct RepArgs {
   double mass[kMaxTypes];
   double dtmax, maxpath;
   int update;
-  double half_skin;
+  float half_skin2;  // (skin/2)^2, less a rounding margin
 };
 
 __device__ inline double block_reduce(double v, bool is_max) {
@@ -354,18 +381,16 @@ __global__ void k_rep_dt(RepArgs a) {
   a.dt[r] = nd;
 }
 
-// Rebuild replica r now if one more step could carry an atom past skin/2:
-// largest displacement since its build plus 1.5x the largest speed times
-// its time step (at least maxpath).  The displacement maximum is cleared for
-// the next drift to fill.
+// Rebuild replica r in this step if one of its atoms has moved skin/2 or
+// more since the replica's last build.  The check runs after every drift,
+// before the force pass, so every pair inside the cut is on the list the
+// forces use.  The displacement maximum is cleared for the next drift.
 __global__ void k_rep_need(RepArgs a) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.nrep) return;
-  const double d = sqrt((double)__uint_as_float(a.d2[r]));
+  const float d2 = __uint_as_float(a.d2[r]);
   a.d2[r] = 0u;
-  double ahead = 1.5 * sqrt(a.vcur[r]) * a.dt[r];
-  if (a.maxpath > ahead) ahead = a.maxpath;
-  const int nd = d + ahead >= a.half_skin ? 1 : 0;
+  const int nd = d2 >= a.half_skin2 ? 1 : 0;
   a.need[r] = nd;
   if (nd) atomicAdd(a.stat + ST_REBUILDS, 1);
 }
@@ -411,7 +436,7 @@ RepArgs rep_args(Ctx& c) {
   a.need = c.rep_need;
   a.step = c.dstep;
   a.stat = c.dstat;
-  a.half_skin = 0.5 * c.skin;
+  a.half_skin2 = (float)(0.25 * c.skin * c.skin * (1.0 - 1e-4));
   a.n = c.n;
   a.nrep = c.nrep;
   a.has_flags = c.has_flags;
@@ -511,8 +536,6 @@ void batch_build(Ctx& c, bool all) {
                                               c.cell_fill, c.cell_start);
   k_batch_scatter<<<grid_blocks(n, kBlock), kBlock, 0, c.stream>>>(
       c.cellid, n, c.rep, c.rep_need, c.cell_start, c.cell_fill, c.perm);
-  k_batch_cell_sort<<<grid_blocks((long)nc * 64, 256), 256, 0, c.stream>>>(
-      c.perm, c.cell_start, nc, c.cell_rep, c.rep_need);
   BBArgs a{};
   a.x = c.x[c.cur];
   a.type = c.type[c.cur];
@@ -536,7 +559,10 @@ void batch_build(Ctx& c, bool all) {
     a.rl2_lo[p] = (float)(a.rl2[p] * (1.0 - 3e-5));
     a.rl2_hi[p] = (float)(a.rl2[p] * (1.0 + 3e-5));
   }
+  a.qs = c.qs;
   k_batch_snapshot<<<grid_blocks(n, kBlock), kBlock, 0, c.stream>>>(a);
+  k_batch_cell_sort<<<grid_blocks((long)nc * 64, 256), 256, 0, c.stream>>>(
+      c.perm, c.cell_start, nc, c.cell_rep, c.rep_need, c.q, c.flag[c.cur], a.has_flags, c.qs);
   k_batch_build<<<grid_blocks(n, kBlock), kBlock, 0, c.stream>>>(a);
   OMDG_HIP(hipGetLastError());
 }

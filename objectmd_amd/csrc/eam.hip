@@ -341,9 +341,10 @@ void eam_forces(Ctx& c, bool energy) {
   a.embed = tab_view(c, c.tab_index[OMDG_TAB_EMBED][0][0]);
   if (!c.eam_pack_ok) build_pack(c, tp, te);
   a.pack = c.eam_pack;
-  // cut radii: table max ranges (team.cpp:78, team.cpp:212)
-  const double ce = a.edens.r0d + (double)(a.edens.n - 1) * a.edens.drd;
-  const double cp = a.phi.r0d + (double)(a.phi.n - 1) * a.phi.drd;
+  // cut radii: table ranges (team.cpp:78, team.cpp:212) up to the end of
+  // their non-zero part
+  const double ce = a.edens.rcut;
+  const double cp = a.phi.rcut;
   a.cut_e2 = (float)(ce * ce);
   a.cut_p2 = (float)(cp * cp);
   a.hx = (float)c.grid.h[0];

@@ -173,7 +173,8 @@ __global__ void __launch_bounds__(kBlock) k_kick_drift(IntArgs a) {
       const float ex = qn.x - qb.x, ey = qn.y - qb.y, ez = qn.z - qb.z;
       disp2 = ex * ex + ey * ey + ez * ez;
     }
-    if (move && disp2 > a.half_skin2) atomicOr(a.stat + ST_ERR, ERR_SKIN);
+    // replicas check every step and rebuild before the force pass
+    if (!BATCH && move && disp2 > a.half_skin2) atomicOr(a.stat + ST_ERR, ERR_SKIN);
     if (!move) disp2 = 0.f;
   }
   if (BATCH) {

@@ -72,15 +72,33 @@ __global__ void k_wrap_aabb(double* __restrict__ x, int n, double3 lo, double3 L
     }
 }
 
-__global__ void k_aabb_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
+// one workgroup of 256 folds the per-block boxes
+__global__ void __launch_bounds__(256) k_aabb_final(const double* __restrict__ part, int nb,
+                                                    double* __restrict__ out) {
   double r[6] = {1e308, 1e308, 1e308, -1e308, -1e308, -1e308};
-  for (int b = 0; b < nb; b++)
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
     for (int d = 0; d < 3; d++) {
       r[d] = fmin(r[d], part[b * 6 + d]);
       r[3 + d] = fmax(r[3 + d], part[b * 6 + 3 + d]);
     }
-  for (int k = 0; k < 6; k++) out[k] = r[k];
+  for (int o = 32; o > 0; o >>= 1)
+    for (int d = 0; d < 3; d++) {
+      r[d] = fmin(r[d], __shfl_xor(r[d], o, 64));
+      r[3 + d] = fmax(r[3 + d], __shfl_xor(r[3 + d], o, 64));
+    }
+  __shared__ double sh[4][6];
+  const int w = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 6; k++) sh[w][k] = r[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int v = 1; v < 4; v++)
+      for (int d = 0; d < 3; d++) {
+        r[d] = fmin(r[d], sh[v][d]);
+        r[3 + d] = fmax(r[3 + d], sh[v][3 + d]);
+      }
+    for (int k = 0; k < 6; k++) out[k] = r[k];
+  }
 }
 
 struct GridDev {
@@ -442,7 +460,7 @@ void nbr_build(Ctx& c) {
                                       make_double3(c.L[0], c.L[1], c.L[2]), c.pbc, part, c.dstat);
   double bb[6] = {0, 0, 0, 0, 0, 0};
   if (c.pbc != 7) {  // free dimensions need the bounding box on the host
-    k_aabb_final<<<1, 64, 0, c.stream>>>(part, nb, c.aabb);
+    k_aabb_final<<<1, 256, 0, c.stream>>>(part, nb, c.aabb);
     OMDG_HIP(hipMemcpyAsync(bb, c.aabb, sizeof bb, hipMemcpyDeviceToHost, c.stream));
     OMDG_HIP(hipStreamSynchronize(c.stream));
   }

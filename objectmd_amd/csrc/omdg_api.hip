@@ -127,6 +127,7 @@ TabView tab_view(const Ctx& c, int tid) {
   v.inv_dr = t.inv_dr_f;
   v.r0d = t.r0;
   v.drd = t.dr;
+  v.rcut = t.rcut;
   v.c32 = reinterpret_cast<const float4*>(c.tab32) + t.off;
   v.c64 = reinterpret_cast<const double4*>(c.tab64) + t.off;
   return v;
@@ -282,7 +283,7 @@ static void batch_radii(Ctx& c) {
         if (!any_tab || (i == 0 && j == 0)) cut = (double)c.sw.a * (double)c.sw.sigma;
         if (tp >= 0 && !(i == 0 && j == 0)) {
           const DevTable& t = c.tables[tp];
-          cut = t.r0 + (double)(t.n - 1) * t.dr;
+          cut = t.rcut;
         }
       } else {
         cut = c.cut;
@@ -370,10 +371,7 @@ static void compute_forces(Ctx& c, bool energy) {
 
 static void update_cut(Ctx& c) {
   double cut = 0;
-  auto tmax = [&](int tid) {
-    const DevTable& t = c.tables[tid];
-    return t.r0 + (double)(t.n - 1) * t.dr;
-  };
+  auto tmax = [&](int tid) { return c.tables[tid].rcut; };
   if (c.potential == OMDG_POT_PAIR || c.potential == OMDG_POT_EAM) {
     for (int a = 0; a < kMaxTypes; a++)
       for (int b = 0; b < kMaxTypes; b++) {
@@ -479,6 +477,7 @@ int omdg_destroy(omdg_t h) {
   dfree(c.bond_b);
   dfree(c.bond_p);
   dfree(c.near);
+  dfree(c.gsum);
   dfree(c.near_n);
   dfree(c.cell_start);
   dfree(c.cell_fill);
@@ -500,6 +499,7 @@ int omdg_destroy(omdg_t h) {
   dfree(c.dstep);
   dfree(c.cell_rep);
   dfree(c.scan_part);
+  dfree(c.qs);
   dfree(c.rep_need);
   dfree(c.rep_d2);
   dfree(c.rep_vcur);
@@ -692,6 +692,14 @@ int omdg_add_table(omdg_t h, int kind, int ta, int tb, int n, double dr, double 
     t.inv_dr_f = (float)(1.0 / dr);
     t.r0_f = (float)r0;
     t.off = (int)(c.host_c64.size() / 4);
+    // interaction range: up to the end of the last interval whose spline
+    // has a coefficient above 1e-30 (a tail of zero samples contributes
+    // nothing an FP32 or FP64 force sum can hold)
+    int last = 0;
+    for (int i = 0; i + 1 < n; i++)
+      for (int k = 0; k < 4; k++)
+        if (std::fabs(abcd[4 * (size_t)i + k]) > 1e-30) last = i;
+    t.rcut = r0 + (double)(last + 1) * dr;
     c.host_c64.insert(c.host_c64.end(), abcd.begin(), abcd.end());
     int id = (int)c.tables.size();
     c.tables.push_back(t);
@@ -972,6 +980,62 @@ int omdg_set_modifies(omdg_t h, double dt_max, double maxpath, int update, int c
   });
 }
 
+}  // extern "C"
+
+// Slot order of replica atoms: caller order between replicas, and inside a
+// replica the order of 4 A cells (z, then y, then x; caller order within a
+// cell), so that neighbours sit close in memory.  The slot -> caller index
+// map (id) undoes it for every copy across the boundary.
+static void order_replicas(Ctx& c, const int* off, int nrep) {
+  const int n = c.n;
+  std::vector<int> id(n), ty(n), fl(n), perm(n);
+  std::vector<double> x(3 * (size_t)n), v(3 * (size_t)n);
+  OMDG_HIP(hipMemcpy(id.data(), c.id[c.cur], sizeof(int) * n, hipMemcpyDeviceToHost));
+  OMDG_HIP(hipMemcpy(ty.data(), c.type[c.cur], sizeof(int) * n, hipMemcpyDeviceToHost));
+  OMDG_HIP(hipMemcpy(fl.data(), c.flag[c.cur], sizeof(int) * n, hipMemcpyDeviceToHost));
+  OMDG_HIP(hipMemcpy(x.data(), c.x[c.cur], sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+  OMDG_HIP(hipMemcpy(v.data(), c.v[c.cur], sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+  // caller index -> current slot
+  std::vector<int> slot(n);
+  for (int s = 0; s < n; s++) slot[id[s]] = s;
+  const double edge = 4.0;
+  std::vector<std::pair<long, int>> key;
+  for (int r = 0; r < nrep; r++) {
+    double mn[3] = {1e308, 1e308, 1e308};
+    for (int i = off[r]; i < off[r + 1]; i++)
+      for (int d = 0; d < 3; d++) mn[d] = std::min(mn[d], x[(size_t)d * n + slot[i]]);
+    key.clear();
+    for (int i = off[r]; i < off[r + 1]; i++) {
+      long k[3];
+      for (int d = 0; d < 3; d++)
+        k[d] = std::min((long)((x[(size_t)d * n + slot[i]] - mn[d]) / edge), (1L << 21) - 1);
+      key.emplace_back((k[2] << 42) | (k[1] << 21) | k[0], i);
+    }
+    std::stable_sort(key.begin(), key.end());
+    for (size_t u = 0; u < key.size(); u++) perm[off[r] + u] = key[u].second;  // caller index
+  }
+  std::vector<int> id2(n), ty2(n), fl2(n);
+  std::vector<double> x2(3 * (size_t)n), v2(3 * (size_t)n);
+  for (int s = 0; s < n; s++) {
+    const int o = slot[perm[s]];
+    id2[s] = perm[s];
+    ty2[s] = ty[o];
+    fl2[s] = fl[o];
+    for (int d = 0; d < 3; d++) {
+      x2[(size_t)d * n + s] = x[(size_t)d * n + o];
+      v2[(size_t)d * n + s] = v[(size_t)d * n + o];
+    }
+  }
+  c.cur = 0;
+  OMDG_HIP(hipMemcpy(c.id[0], id2.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  OMDG_HIP(hipMemcpy(c.type[0], ty2.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  OMDG_HIP(hipMemcpy(c.flag[0], fl2.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  OMDG_HIP(hipMemcpy(c.x[0], x2.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  OMDG_HIP(hipMemcpy(c.v[0], v2.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+}
+
+extern "C" {
+
 int omdg_set_replicas(omdg_t h, int nrep, const int* off, const double* lo3, const double* hi3,
                       const int* pbc) {
   if (!h || nrep < 1 || !off || !lo3 || !hi3 || !pbc) {
@@ -984,7 +1048,7 @@ int omdg_set_replicas(omdg_t h, int nrep, const int* off, const double* lo3, con
     if (off[0] != 0 || off[nrep] != c.n) throw ApiFail(OMDG_EINVAL, "replica offsets must cover 0..n");
     for (int r = 0; r < nrep; r++)
       if (off[r + 1] <= off[r]) throw ApiFail(OMDG_EINVAL, "empty replica");
-    if (c.list_valid && c.cur != 0) throw ApiFail(OMDG_EINVAL, "set replicas before the first list build");
+    order_replicas(c, off, nrep);
     c.nrep = nrep;
     c.rep_off.assign(off, off + nrep + 1);
     c.rep_lo.assign(lo3, lo3 + 3 * (size_t)nrep);
@@ -1002,6 +1066,7 @@ int omdg_set_replicas(omdg_t h, int nrep, const int* off, const double* lo3, con
     dalloc(c.rep_vcur, nrep);
     dalloc(c.rep_d2, nrep);
     dalloc(c.rep_need, nrep);
+    dalloc(c.qs, c.n);
     if (!c.dstep) dalloc(c.dstep, 1);
     OMDG_HIP(hipMemcpy(c.rep, rid.data(), sizeof(int) * c.n, hipMemcpyHostToDevice));
     OMDG_HIP(hipMemcpy(c.rep_offd, off, sizeof(int) * (nrep + 1), hipMemcpyHostToDevice));

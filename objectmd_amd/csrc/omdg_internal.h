@@ -41,6 +41,7 @@ struct DevTable {
   int n = 0;
   int rmult = 0;
   double dr = 0, r0 = 0;
+  double rcut = 0;  // interaction range (end of the non-zero part)
   float inv_dr_f = 0, dr_f = 0, r0_f = 0;
   int off = -1;  // offset (in knots) into the packed coefficient arrays
 };
@@ -51,6 +52,7 @@ struct TabView {
   int rmult;
   float r0, dr, inv_dr;
   double r0d, drd;
+  double rcut;
   const float4* c32;   // {a,b,c,d}
   const double4* c64;  // {a,b,c,d}
 };
@@ -112,6 +114,7 @@ struct Ctx {
   float* f32 = nullptr;   // 3 planes (FP32 force mode)
   float4* q = nullptr;    // cell-relative position, w = type bits
   float4* qb = nullptr;   // q at the last build
+  float4* qs = nullptr;   // replicas: packed list-build candidates in cell order
   double* xb = nullptr;   // FP64 positions at the last build (FP64 mode)
   int* cellid = nullptr;
   int* cellid2 = nullptr;
@@ -127,6 +130,7 @@ struct Ctx {
   float* bond_p = nullptr;  // Tersoff per-slot zeta prefactor
   size_t cap_bond = 0;
   float4* near = nullptr;   // SW near lists (displacement, entry) per slot
+  float4* gsum = nullptr;   // SW per-leg three-body force and energy share
   int* near_n = nullptr;
   size_t cap_near = 0;
   int cap_atoms = 0;

@@ -203,7 +203,7 @@ void pair_forces(Ctx& c, bool energy) {
       }
       a.tab[slot] = (signed char)found;
       const TabView& t = a.tabs[found];
-      a.cut[slot] = t.r0d + (double)(t.n - 1) * t.drd;
+      a.cut[slot] = t.rcut;
     }
   const int nb = grid_blocks(c.n, kBlock);
   if (c.fp64) {

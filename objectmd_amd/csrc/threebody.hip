@@ -2,18 +2,20 @@
 // accumulates its own force; no atomics, deterministic).  FP32 arithmetic on
 // the cell-relative snapshots with Kahan-compensated per-atom sums.
 //
-// Stillinger-Weber (src/gadget/sw.cpp), two passes:
-//   pass 1    two-body terms: ZBL core below sx0, polynomial splice below
-//             sx1, SW beyond (sw.cpp:118-189, 240-252); pairs of other type
+// Stillinger-Weber (src/gadget/sw.cpp), three passes:
+//   pass 1    list scan: each atom's near list (its SW neighbours inside
+//             a*sigma with their displacement vectors); pairs of other type
 //             combinations with a PAIR table use the table (TForcePair,
-//             tpair.h:62-71).  Writes each atom's near list: its SW
-//             neighbours inside a*sigma with their displacement vectors.
-//   pass 2    three-body h(j,i,k) for every centre i and unordered pair
-//             {j,k} of its near neighbours (sw.cpp:191-236, 254-273).  Atom
-//             l gathers the triplets where it is the centre and those where
-//             it is a leg (centre c = a near neighbour of l, other leg k
-//             from c's near list); each triplet gives a third of its energy
-//             to each atom, as the reference does.
+//             tpair.h:62-71); then the two-body terms of the near list: ZBL
+//             core below sx0, polynomial splice below sx1, SW beyond
+//             (sw.cpp:118-189, 240-252)
+//   pass 2    three-body h(j,i,k) for every centre i and pair {j,k} of its
+//             near neighbours (sw.cpp:191-236, 254-273), evaluated per centre:
+//             the centre's force, and per leg the force and energy share it
+//             receives from this centre's triplets
+//   pass 3    each atom adds the leg shares stored by its near neighbours
+//             for it; every triplet gives a third of its energy to each of
+//             its atoms, as the reference does
 // Tersoff (PRB 37 6991 with the cutoff of include/potential/tersoff.h:247-257)
 //   pass 1    per bond (i,j): zeta_ij, b_ij, the pair energy 1/2 V_ij and
 //             P_ij = 1/2 fc fA db/dzeta stored per list slot
@@ -21,6 +23,8 @@
 //             zeta derivatives of its own bonds, of the bonds (j,l) and of the
 //             bonds (j,k) in whose zeta it appears.
 #include <hip/hip_runtime.h>
+
+#include <cmath>
 
 #include "omdg_internal.h"
 #include "spline_dev.h"
@@ -41,7 +45,8 @@ struct TBArgs {
   float* vir;
   float* bond_b;  // Tersoff: b_ij per list slot
   float* bond_p;  // Tersoff: P_ij per list slot
-  float4* near;   // SW: near list, kMaxNear rows of n (x_j - x_l, entry bits)
+  float4* near;   // SW: near list, kMaxNear slot rows of n (x_j - x_l, entry bits)
+  float4* gsum;   // SW: per near entry, force on that leg from the centre's triplets
   int* near_n;    // SW: near-list length per atom
   int* err;
   int n;
@@ -56,6 +61,7 @@ struct TBArgs {
   float c1, c2, c3, c4, c5, c6, c7, c8, c9, c10;
   float zc, ze[4];
   int sw_type;  // atom type the SW parameters apply to (-1: all)
+  int pi, qi;   // SW exponents p, q when small integers (-1 otherwise)
   signed char tab[kMaxTypes * kMaxTypes];
   float tcut2[kMaxTypes * kMaxTypes];
   TabView tabs[kMaxTypes * kMaxTypes];
@@ -137,9 +143,49 @@ __device__ inline float4 off_of(uint32_t code, const float4& h) {
                      (float)(c / 9 - 1) * h.z, 0.f);
 }
 
-// Pass 1: two-body terms (SW with its ZBL core and splice, pair tables for
-// the other type pairs) and the compact near list of every atom: the SW
-// neighbours inside a*sigma as {x_j - x_l, entry} in slot-major float4 rows.
+__device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// One atom's entries of a slot-major per-atom array: entry k of atom i at
+// k * n + i, so a wavefront reading entry k of consecutive atoms reads one
+// contiguous run.
+struct Col {
+  float4* p;
+  size_t n;
+  __device__ float4& operator[](int k) const { return p[(size_t)k * n]; }
+};
+
+// r^-p for the SW exponents: repeated products when p is a small integer
+__device__ inline float pow_neg(float r, float ir, float p, int pi) {
+  if (pi >= 0) {
+    float v = 1.0f;
+    for (int k = 0; k < pi; k++) v *= ir;
+    return v;
+  }
+  return powf(r, -p);
+}
+
+// Two-body SW with the ZBL core and splice; returns fr (force on the first
+// atom = fr * (x_first - x_second)) and the half energy.
+__device__ inline void sw_two_fast(const TBArgs& a, float r, float& fr, float& ehalf) {
+  const SwParams& p = a.sw;
+  if (p.usezbl && r < p.sx1) {
+    sw_two(a, r, fr, ehalf);  // ZBL core and splice (close collisions)
+    return;
+  }
+  const float ir = rcp(r);
+  const float ira = rcp(r - a.rc);
+  const float rp = pow_neg(r, ir, p.p, a.pi), rq = pow_neg(r, ir, p.q, a.qi);
+  const float fex = expf(p.sigma * ira);
+  fr = (a.c1 * rp - a.c2 * rq + (a.c3 * rp - a.c4 * rq) * ira * ira * r) * fex * ir * ir;
+  ehalf = 0.5f * (a.c5 * rp - a.c6 * rq) * fex;
+}
+
+// Pass 1: the list scan and the two-body terms.  Phase A walks the full
+// list: SW neighbours inside a*sigma go to the atom's near list as
+// {x_j - x_l, entry} (slot-major, kMaxNear slots), pairs of other type combinations with a
+// PAIR table take the table (TForcePair, tpair.h:62-71).  Phase B evaluates
+// the SW two-body term of each near neighbour (no divergence from the list
+// scan).
 template <bool ENERGY>
 __global__ void __launch_bounds__(kBlock) k_sw_pair(TBArgs a) {
   const int l = xcd_block() * blockDim.x + threadIdx.x;
@@ -150,6 +196,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_pair(TBArgs a) {
   const float4 ql = a.q[l];
   const int tl = snap_type(ql.w);
   const bool l_sw = a.sw_type < 0 || tl == a.sw_type;
+  const Col nl{a.near + l, (size_t)n};
   Kahan fx, fy, fz;
   float pot = 0.f, vir = 0.f;
   int nn = 0;
@@ -167,15 +214,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_pair(TBArgs a) {
     const bool pair_sw = l_sw && (a.sw_type < 0 || tj == a.sw_type);
     if (pair_sw) {
       if (r2 < a.rc2) {
-        const float r = sqrtf(r2);
-        float fr, eh;
-        sw_two(a, r, fr, eh);
-        // force on l = fr * (x_l - x_j)
-        fx.add(-fr * d.x);
-        fy.add(-fr * d.y);
-        fz.add(-fr * d.z);
-        if (ENERGY) pot += eh;
-        if (nn < kMaxNear) a.near[(size_t)nn * n + l] = make_float4(d.x, d.y, d.z, __uint_as_float(e));
+        if (nn < kMaxNear) nl[nn] = make_float4(d.x, d.y, d.z, __uint_as_float(e));
         else atomicOr(a.err, ERR_NEAR);
         nn++;
       }
@@ -197,7 +236,19 @@ __global__ void __launch_bounds__(kBlock) k_sw_pair(TBArgs a) {
       }
     }
   }
-  a.near_n[l] = nn < kMaxNear ? nn : kMaxNear;
+  if (nn > kMaxNear) nn = kMaxNear;
+  for (int u = 0; u < nn; u++) {
+    const float4 d = nl[u];
+    const float r = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+    float fr, eh;
+    sw_two_fast(a, r, fr, eh);
+    // force on l = fr * (x_l - x_j)
+    fx.add(-fr * d.x);
+    fy.add(-fr * d.y);
+    fz.add(-fr * d.z);
+    if (ENERGY) pot += eh;
+  }
+  a.near_n[l] = nn;
   a.f[l] = fx.s;
   a.f[(size_t)n + l] = fy.s;
   a.f[2 * (size_t)n + l] = fz.s;
@@ -207,56 +258,108 @@ __global__ void __launch_bounds__(kBlock) k_sw_pair(TBArgs a) {
   }
 }
 
-// Pass 2: three-body terms from the near lists.  Atom l gathers the
-// triplets centred at l and those centred at a near neighbour c with l as
-// one leg (the other leg k from c's near list); each gives a third of its
-// energy to l.  The sums continue from the pass-1 forces.
+// Leg data {r, 1/r, 1/(r - a sigma), exp(gamma sigma / (r - a sigma))} of a
+// near entry (recomputed where needed: cheaper than storing it).
+__device__ inline float4 leg_data(const TBArgs& a, const float4& d) {
+  const float r = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+  const float ira = rcp(r - a.rc);
+  return make_float4(r, rcp(r), ira, expf(a.c7 * ira));
+}
+
+// Force on leg j of the SW triplet h(j, i, k) (sw.cpp:191-236) and the
+// triplet energy, from the legs' displacements d1 = x_j - x_i, d2 = x_k - x_i
+// and their leg data {r, 1/r, 1/(r - a sigma), exp(gamma sigma/(r - a sigma))}.
+__device__ inline float sw_leg(const TBArgs& a, const float4& d1, const float4& x1,
+                               const float4& d2, const float4& x2, float g[3]) {
+  const float ir12 = x1.y * x2.y;
+  const float cs = (d1.x * d2.x + d1.y * d2.y + d1.z * d2.z) * ir12;
+  const float dc = cs - a.c10;
+  const float fex = x1.w * x2.w;
+  const float ftheta = a.c9 * fex * dc;
+  const float fp = a.c8 * fex * dc * dc;
+  const float ft12 = ir12 * ftheta;
+  // fp * fi0 * ira0 / r1 with fi0 = gamma sigma ira0, and cos theta ftheta / r1^2
+  const float f1 = fp * a.c7 * x1.z * x1.z * x1.y + cs * ftheta * x1.y * x1.y;
+  g[0] = d1.x * f1 - d2.x * ft12;
+  g[1] = d1.y * f1 - d2.y * ft12;
+  g[2] = d1.z * f1 - d2.z * ft12;
+  return fp;
+}
+
+// Pass 2: the triplets centred at each atom i, once per ordered leg pair
+// (u, w): leg u's force summed over w goes to gsum[i][u] with a third of
+// the triplet energies (the leg's share); the centre takes minus the sum of
+// its legs' forces and a third of each triplet's energy.
 template <bool ENERGY>
-__global__ void __launch_bounds__(kBlock) k_sw_three(TBArgs a) {
+__global__ void __launch_bounds__(kBlock) k_sw_centre(TBArgs a) {
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int n = a.n;
+  const int nn = a.near_n[i];
+  if (nn < 2) return;
+  const Col ni{a.near + i, (size_t)n}, gi{a.gsum + i, (size_t)n};
+  Kahan fx, fy, fz;
+  fx.s = a.f[i];
+  fy.s = a.f[(size_t)n + i];
+  fz.s = a.f[2 * (size_t)n + i];
+  float esum = 0.f;
+  for (int u = 0; u < nn; u++) {
+    const float4 du = ni[u], xu = leg_data(a, du);
+    float gx = 0.f, gy = 0.f, gz = 0.f, e = 0.f;
+    for (int w = 0; w < nn; w++) {
+      if (w == u) continue;
+      float g[3];
+      const float4 dw = ni[w];
+      e += sw_leg(a, du, xu, dw, leg_data(a, dw), g);
+      gx += g[0];
+      gy += g[1];
+      gz += g[2];
+    }
+    gi[u] = make_float4(gx, gy, gz, e * (1.0f / 3.0f));
+    fx.add(-gx);
+    fy.add(-gy);
+    fz.add(-gz);
+    esum += e;
+  }
+  a.f[i] = fx.s;
+  a.f[(size_t)n + i] = fy.s;
+  a.f[2 * (size_t)n + i] = fz.s;
+  // every unordered triplet was summed twice: a third of it to the centre
+  if (ENERGY) a.pot[i] += esum * (1.0f / 6.0f);
+}
+
+// Pass 3: atom l collects, from each near neighbour c, the force and energy
+// share of the triplets centred at c with l as a leg (gsum[c][slot of l]).
+template <bool ENERGY>
+__global__ void __launch_bounds__(kBlock) k_sw_legs(TBArgs a) {
   const int l = xcd_block() * blockDim.x + threadIdx.x;
   if (l >= a.n) return;
   const int n = a.n;
   const int nn = a.near_n[l];
   if (nn == 0) return;
+  const Col nl{a.near + l, (size_t)n};
   Kahan fx, fy, fz;
   fx.s = a.f[l];
   fy.s = a.f[(size_t)n + l];
   fz.s = a.f[2 * (size_t)n + l];
   float pot = 0.f;
-  const float third = 1.0f / 3.0f;
   for (int u = 0; u < nn; u++) {
-    const float4 nu = a.near[(size_t)u * n + l];
-    const float d1[3] = {nu.x, nu.y, nu.z};
-    const float r1 = sqrtf(nu.x * nu.x + nu.y * nu.y + nu.z * nu.z);
-    // triplets centred at l: legs u < w
-    for (int w = u + 1; w < nn; w++) {
-      const float4 nw = a.near[(size_t)w * n + l];
-      const float d2[3] = {nw.x, nw.y, nw.z};
-      SwTerm t;
-      sw_three(a, d1, r1, d2, sqrtf(nw.x * nw.x + nw.y * nw.y + nw.z * nw.z), t);
-      fx.add(-(t.gj[0] + t.gk[0]));
-      fy.add(-(t.gj[1] + t.gk[1]));
-      fz.add(-(t.gj[2] + t.gk[2]));
-      if (ENERGY) pot += third * t.e;
-    }
-    // triplets centred at c = u's atom; leg l seen from c: x_l - x_c
-    const int c = (int)(__float_as_uint(nu.w) & kSlotMask);
-    const float e1[3] = {-nu.x, -nu.y, -nu.z};
+    const float4 du = nl[u];
+    const int c = (int)(__float_as_uint(du.w) & kSlotMask);
     const int nc = a.near_n[c];
-    for (int w = 0; w < nc; w++) {
-      const float4 nk = a.near[(size_t)w * n + c];  // x_k - x_c
-      if ((int)(__float_as_uint(nk.w) & kSlotMask) == l) {
-        const float ex = nk.x - e1[0], ey = nk.y - e1[1], ez = nk.z - e1[2];
-        if (ex * ex + ey * ey + ez * ez < 1e-4f) continue;  // the same image of l
-      }
-      const float d2[3] = {nk.x, nk.y, nk.z};
-      SwTerm tt;
-      sw_three(a, e1, r1, d2, sqrtf(nk.x * nk.x + nk.y * nk.y + nk.z * nk.z), tt);
-      // l as leg j of the triplet (c; l, k)
-      fx.add(tt.gj[0]);
-      fy.add(tt.gj[1]);
-      fz.add(tt.gj[2]);
-      if (ENERGY) pot += third * tt.e;
+    if (nc < 2) continue;
+    const Col ncl{a.near + c, (size_t)n};
+    for (int s = 0; s < nc; s++) {
+      const float4 dk = ncl[s];  // x_k - x_c
+      if ((int)(__float_as_uint(dk.w) & kSlotMask) != l) continue;
+      const float ex = dk.x + du.x, ey = dk.y + du.y, ez = dk.z + du.z;
+      if (ex * ex + ey * ey + ez * ez >= 1e-4f) continue;  // another image of l
+      const float4 g = a.gsum[(size_t)s * n + c];
+      fx.add(g.x);
+      fy.add(g.y);
+      fz.add(g.z);
+      if (ENERGY) pot += g.w;
+      break;
     }
   }
   a.f[l] = fx.s;
@@ -573,21 +676,29 @@ void sw_forces(Ctx& c, bool energy) {
         found = used++;
       }
       a.tab[s] = (signed char)found;
-      const double cut = a.tabs[found].r0d + (double)(a.tabs[found].n - 1) * a.tabs[found].drd;
+      const double cut = a.tabs[found].rcut;
       a.tcut2[s] = (float)(cut * cut);
     }
   a.sw_type = any_tab ? 0 : -1;
+  const double pd = p.p, qd = p.q;
+  a.pi = (pd == std::floor(pd) && pd >= 0 && pd <= 16) ? (int)pd : -1;
+  a.qi = (qd == std::floor(qd) && qd >= 0 && qd <= 16) ? (int)qd : -1;
   if ((size_t)c.n * kMaxNear > c.cap_near) {
-    if (c.near) (void)hipFree(c.near);
+    for (float4** pp : {&c.near, &c.gsum}) {
+      if (*pp) (void)hipFree(*pp);
+      *pp = nullptr;
+    }
     if (c.near_n) (void)hipFree(c.near_n);
-    c.near = nullptr;
     c.near_n = nullptr;
     c.cap_near = 0;
-    OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.near), sizeof(float4) * kMaxNear * (size_t)c.n));
+    const size_t bytes = sizeof(float4) * kMaxNear * (size_t)c.n;
+    OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.near), bytes));
+    OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.gsum), bytes));
     OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.near_n), sizeof(int) * (size_t)c.n));
     c.cap_near = (size_t)c.n * kMaxNear;
   }
   a.near = c.near;
+  a.gsum = c.gsum;
   a.near_n = c.near_n;
   const int nb = grid_blocks(c.n, kBlock);
   {
@@ -597,8 +708,13 @@ void sw_forces(Ctx& c, bool energy) {
   }
   {
     TimerScope ts(c, TG_FORCE);
-    if (energy) k_sw_three<true><<<nb, kBlock, 0, c.stream>>>(a);
-    else k_sw_three<false><<<nb, kBlock, 0, c.stream>>>(a);
+    if (energy) {
+      k_sw_centre<true><<<nb, kBlock, 0, c.stream>>>(a);
+      k_sw_legs<true><<<nb, kBlock, 0, c.stream>>>(a);
+    } else {
+      k_sw_centre<false><<<nb, kBlock, 0, c.stream>>>(a);
+      k_sw_legs<false><<<nb, kBlock, 0, c.stream>>>(a);
+    }
   }
   OMDG_HIP(hipGetLastError());
 }

@@ -74,7 +74,11 @@ def test_batch_lists_bit_exact():
     swp, pair = oracle_pot()
     f32 = lambda v: float(np.float32(v))
     rc_sw = f32(swp.a) * f32(swp.sigma)
-    rc_pair = pair.t.r0 + (pair.t.n - 1) * pair.t.dr
+    # the table's range ends with its last interval holding a coefficient
+    # above 1e-30 (the zero tail beyond 3.2 A holds nothing)
+    last = np.nonzero((np.abs(pair.coef[:-1]) > 1e-30).any(1))[0][-1]
+    rc_pair = pair.t.r0 + (last + 1) * pair.t.dr
+    assert 3.3 < rc_pair < 3.6
     rl2 = {(0, 0): f32((rc_sw + SKIN) ** 2), (0, 1): f32((rc_pair + SKIN) ** 2),
            (1, 0): f32((rc_pair + SKIN) ** 2), (1, 1): 0.0}
     for r in range(len(b["off"]) - 1):
@@ -83,7 +87,8 @@ def test_batch_lists_bit_exact():
         L = hi - lo
         x = wrap_rep(b["x"][i0:i1], lo, hi)
         tid = b["tid"][i0:i1]
-        oc, ocode = O.neighbors(x, lo, L, 3, (rc_pair + SKIN) * (1 + 1e-6), maxn=160)
+        oc, ocode = O.neighbors(x, lo, L, 3, (max(rc_sw, rc_pair) + SKIN) * (1 + 1e-6),
+                                maxn=160)
         for i in range(i1 - i0):
             keep = []
             for c in ocode[i, :oc[i]]:

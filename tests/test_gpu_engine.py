@@ -91,10 +91,11 @@ def test_neighbor_list_bit_exact(name):
     eng = om.Engine()
     eng.set_box(lo, hi, pbc)
     eng.upload_atoms(x, mass=[1.0])
-    # a pair table whose max range is the list radius minus the skin
+    # a pair table whose range is the list radius minus the skin (constant
+    # and non-zero, so its range is not cut back to a zero tail)
     cut = rlist - 0.25
     r = np.linspace(0.5, cut, 200)
-    eng.add_table(om.TAB_PAIR, 0, 0, np.zeros_like(r), r[1] - r[0], 0.5)
+    eng.add_table(om.TAB_PAIR, 0, 0, np.ones_like(r), r[1] - r[0], 0.5)
     eng.set_potential(om.POT_PAIR, fp64=True)
     eng.set_neighbor(rlist - (0.5 + (r[1] - r[0]) * 199), 10, 0)
     eng.neighbor_build()
