@@ -39,7 +39,7 @@ enum { OMDG_X = 0, OMDG_V = 1, OMDG_F = 2, OMDG_POT = 3, OMDG_VIR = 4,
 enum { OMDG_TAB_PAIR = 0, OMDG_TAB_EDENS = 1, OMDG_TAB_EMBED = 2 };
 /* potentials */
 enum { OMDG_POT_NONE = 0, OMDG_POT_PAIR = 1, OMDG_POT_EAM = 2,
-       OMDG_POT_SW = 3, OMDG_POT_TERSOFF = 4 };
+       OMDG_POT_SW = 3, OMDG_POT_TERSOFF = 4, OMDG_POT_SURROGATE = 5 };
 /* atom flags: active as FLAG_ACTIVE (base.h:230); FIXED marks the atoms of a
  * CoordClamp group, DAMP those of a ForceDamper group, LIMIT those of a
  * RangeLimiter group */
@@ -96,6 +96,30 @@ int omdg_set_sw(omdg_t ctx, const double* p19);
  * A B lambda_1 lambda_2 lambda_3 alpha beta n c d h R D (13 values);
  * cutoff function on [R-D, R+D] (tersoff.h:247-257). */
 int omdg_set_tersoff(omdg_t ctx, const double* p13);
+
+/* Learned surrogate of the Tersoff potential (no counterpart in the
+ * reference; selected with OMDG_POT_SURROGATE, in place of the bond-order
+ * force of tersoff.h:259-365).  Per atom a descriptor G of 24 radial and 8
+ * angular symmetry functions,
+ *   G_k = sum_j exp(-eta_r (r_ij - mu0 - k dmu)^2) fc(r_ij)
+ *   G_m = sum_{j<k} A_m(cos t_jik) exp(-eta_a (r_ij^2 + r_ik^2)) fc(r_ij) fc(r_ik)
+ * with A_m(c) = exp(-q_m (c - p_m)^2) (kind 1) or 2^(1-q_m) (1 + p_m c)^q_m
+ * (kind 0, q_m integer), fc(r) = (cos(pi r/rc) + 1)/2, and an energy
+ * E_i = MLP((G - mean)/std) with layers 32-64-64-1 and SiLU activations,
+ * run on the matrix cores in BF16 with FP32 sums; forces by the chain rule.
+ * desc (22): rc eta_r mu0 dmu eta_a kind, then (p, q) x 8.
+ * norm: mean[32], std[32], lo[32], hi[32] (the training descriptors' box).
+ * weights (FP32, converted to BF16): W1[64][32] b1[64] W2[64][64] b2[64]
+ * w3[64] b3.  A step whose largest descriptor lies more than
+ * gauge_threshold box widths outside the box is computed with Tersoff
+ * (parameters from omdg_set_tersoff) instead; the choice is made on the
+ * device. */
+int omdg_set_surrogate(omdg_t ctx, const double* desc, const double* norm,
+                       const float* weights, double gauge_threshold);
+/* Force evaluations with the surrogate so far, how many fell back to
+ * Tersoff, and the largest gauge seen (each may be NULL). */
+int omdg_surrogate_stats(omdg_t ctx, int* steps, int* fallback_steps,
+                         double* gauge_max);
 
 /* Verlet list control.  Replaces VerletList/VerletListFull parameters
  * verlet.radtole / verlet.n_mean (VerletList.cpp:46-52) and the refresh

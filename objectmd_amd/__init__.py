@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_PKG, "libomdg.so")
 OK, EINVAL, EHIP, ERANGE_HI, ERANGE_LO, EOVERLAP, ENBR, ENODEV = range(8)
 X, V, F, POT, VIR, RHO = range(6)
 TAB_PAIR, TAB_EDENS, TAB_EMBED = range(3)
-POT_PAIR, POT_EAM, POT_SW, POT_TERSOFF = 1, 2, 3, 4
+POT_PAIR, POT_EAM, POT_SW, POT_TERSOFF, POT_SURROGATE = 1, 2, 3, 4, 5
 FLAG_ACTIVE, FLAG_FIXED, FLAG_DAMP, FLAG_LIMIT = 1, 8, 16, 32
 
 # every entry point declared in include/omdg.h
@@ -23,6 +23,7 @@ ABI_SYMBOLS = (
     "omdg_create", "omdg_destroy", "omdg_last_error", "omdg_set_box",
     "omdg_upload_atoms", "omdg_set_state", "omdg_download", "omdg_add_table",
     "omdg_load_table", "omdg_set_potential", "omdg_set_sw", "omdg_set_tersoff",
+    "omdg_set_surrogate", "omdg_surrogate_stats",
     "omdg_set_neighbor", "omdg_neighbor_build", "omdg_neighbor_export",
     "omdg_compute", "omdg_zero_forces", "omdg_step", "omdg_set_modifies",
     "omdg_set_replicas", "omdg_batch_run", "omdg_timing",
@@ -63,6 +64,8 @@ def load_library(path=LIB_PATH):
         "omdg_set_potential": [P, ctypes.c_int, ctypes.c_int],
         "omdg_set_sw": [P, dp],
         "omdg_set_tersoff": [P, dp],
+        "omdg_set_surrogate": [P, dp, dp, ctypes.POINTER(ctypes.c_float), ctypes.c_double],
+        "omdg_surrogate_stats": [P, ip, ip, dp],
         "omdg_set_neighbor": [P, ctypes.c_double, ctypes.c_int, ctypes.c_int],
         "omdg_neighbor_build": [P],
         "omdg_neighbor_export": [P, ctypes.c_int, ip, ip],
@@ -173,6 +176,26 @@ class Engine:
         p = np.ascontiguousarray(p13, dtype=np.float64)
         assert p.size == 13
         self._check(self.lib.omdg_set_tersoff(self.h, _dptr(p)))
+
+    def set_surrogate(self, desc, norm, weights, gauge_threshold):
+        """Learned Tersoff surrogate: 22 descriptor parameters, mean/std/lo/hi
+        of the 32 descriptors, the 6337 MLP weights, the fallback threshold."""
+        d = np.ascontiguousarray(desc, dtype=np.float64)
+        nm = np.ascontiguousarray(norm, dtype=np.float64)
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        assert d.size == 22 and nm.size == 128 and w.size == 6337
+        self._check(self.lib.omdg_set_surrogate(
+            self.h, _dptr(d), _dptr(nm), w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+            float(gauge_threshold)))
+
+    def surrogate_stats(self):
+        """-> (evaluations, fallback evaluations, largest gauge)."""
+        st = ctypes.c_int()
+        fb = ctypes.c_int()
+        g = ctypes.c_double()
+        self._check(self.lib.omdg_surrogate_stats(self.h, ctypes.byref(st), ctypes.byref(fb),
+                                                  ctypes.byref(g)))
+        return st.value, fb.value, g.value
 
     def set_neighbor(self, skin, check_every=10, maxnbr=0):
         self._check(self.lib.omdg_set_neighbor(self.h, float(skin), int(check_every),

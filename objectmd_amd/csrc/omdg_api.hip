@@ -364,6 +364,7 @@ static void compute_forces(Ctx& c, bool energy) {
     }
     case OMDG_POT_SW: sw_forces(c, energy); break;
     case OMDG_POT_TERSOFF: tersoff_forces(c, energy); break;
+    case OMDG_POT_SURROGATE: surrogate_forces(c, energy); break;
     default: throw ApiFail(OMDG_EINVAL, "no potential selected");
   }
   c.forces_valid = true;
@@ -392,6 +393,9 @@ static void update_cut(Ctx& c) {
       }
   } else if (c.potential == OMDG_POT_TERSOFF) {
     cut = (double)c.ters.R + (double)c.ters.D;
+  } else if (c.potential == OMDG_POT_SURROGATE) {
+    // the descriptor's cut, or the Tersoff cut of the fallback steps
+    cut = std::max(c.sg.desc[0], (double)c.ters.R + (double)c.ters.D);
   }
   if (cut != c.cut) {
     c.cut = cut;
@@ -500,6 +504,7 @@ int omdg_destroy(omdg_t h) {
   dfree(c.cell_rep);
   dfree(c.scan_part);
   dfree(c.qs);
+  surrogate_free(c);
   dfree(c.rep_need);
   dfree(c.rep_d2);
   dfree(c.rep_vcur);
@@ -728,7 +733,7 @@ int omdg_load_table(omdg_t h, int kind, int ta, int tb, const char* path, const 
 }
 
 int omdg_set_potential(omdg_t h, int potential, int fp64) {
-  if (!h || potential < OMDG_POT_PAIR || potential > OMDG_POT_TERSOFF) {
+  if (!h || potential < OMDG_POT_PAIR || potential > OMDG_POT_SURROGATE) {
     set_error("bad potential");
     return OMDG_EINVAL;
   }
@@ -746,8 +751,13 @@ int omdg_set_potential(omdg_t h, int potential, int fp64) {
         for (int b : a) any |= b >= 0;
       if (!any) throw ApiFail(OMDG_EINVAL, "pair potential needs a PAIR table");
     }
-    if ((potential == OMDG_POT_SW || potential == OMDG_POT_TERSOFF) && fp64)
+    if ((potential == OMDG_POT_SW || potential == OMDG_POT_TERSOFF ||
+         potential == OMDG_POT_SURROGATE) && fp64)
       throw ApiFail(OMDG_EINVAL, "three-body kernels compute FP32 forces");
+    if (potential == OMDG_POT_SURROGATE && (!c.sg.loaded || !c.ters_set))
+      throw ApiFail(OMDG_EINVAL,
+                    "the surrogate needs its model (omdg_set_surrogate) and the Tersoff "
+                    "parameters it falls back to (omdg_set_tersoff)");
     c.potential = potential;
     c.fp64 = fp64 ? 1 : 0;
     update_cut(c);
@@ -785,7 +795,58 @@ int omdg_set_tersoff(omdg_t h, const double* p) {
     t.lambda3 = (float)p[4]; t.alpha = (float)p[5]; t.beta = (float)p[6]; t.n = (float)p[7];
     t.c = (float)p[8]; t.d = (float)p[9]; t.h = (float)p[10]; t.R = (float)p[11];
     t.D = (float)p[12]; t.m = 3;
-    if (c.potential == OMDG_POT_TERSOFF) update_cut(c);
+    c.ters_set = true;
+    if (c.potential == OMDG_POT_TERSOFF || c.potential == OMDG_POT_SURROGATE) update_cut(c);
+  });
+}
+
+int omdg_set_surrogate(omdg_t h, const double* desc, const double* norm, const float* weights,
+                       double gauge_threshold) {
+  if (!h || !desc || !norm || !weights || !(gauge_threshold >= 0.0)) {
+    set_error("bad surrogate arguments");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    OMDG_HIP(hipSetDevice(c.device));
+    if (!(desc[0] > 0.0)) throw ApiFail(OMDG_EINVAL, "descriptor cut must be positive");
+    for (int g = 0; g < kSgDesc; g++) {
+      if (!(norm[kSgDesc + g] > 0.0)) throw ApiFail(OMDG_EINVAL, "descriptor std must be positive");
+      if (!(norm[3 * kSgDesc + g] > norm[2 * kSgDesc + g]))
+        throw ApiFail(OMDG_EINVAL, "gauge box must have hi > lo");
+    }
+    const double kind = desc[5];
+    if (kind != 0.0 && kind != 1.0) throw ApiFail(OMDG_EINVAL, "angular kind must be 0 or 1");
+    for (int m = 0; m < kSgAngular; m++) {
+      const double q = desc[7 + 2 * m];
+      if (kind == 0.0 && !(q >= 1.0 && q <= 64.0 && q == std::floor(q)))
+        throw ApiFail(OMDG_EINVAL, "angular exponent must be an integer in [1, 64]");
+      if (kind == 1.0 && !(q > 0.0)) throw ApiFail(OMDG_EINVAL, "angular width must be positive");
+    }
+    surrogate_upload(c, desc, norm, weights);
+    c.sg.threshold = gauge_threshold;
+    if (c.potential == OMDG_POT_SURROGATE) update_cut(c);
+    c.forces_valid = false;
+  });
+}
+
+int omdg_surrogate_stats(omdg_t h, int* steps, int* fallback_steps, double* gauge_max) {
+  if (!h) {
+    set_error("null context");
+    return OMDG_EINVAL;
+  }
+  return guarded([&] {
+    Ctx& c = h->c;
+    int st[8] = {0};
+    if (c.sg.state) {
+      OMDG_HIP(hipStreamSynchronize(c.stream));
+      OMDG_HIP(hipMemcpy(st, c.sg.state, sizeof st, hipMemcpyDeviceToHost));
+    }
+    if (steps) *steps = st[1];
+    if (fallback_steps) *fallback_steps = st[2];
+    float g;
+    std::memcpy(&g, &st[5], sizeof g);
+    if (gauge_max) *gauge_max = g;
   });
 }
 

@@ -92,6 +92,24 @@ struct TersoffParams {
   int m;
 };
 
+// Learned surrogate: descriptor of kSgRadial + kSgAngular symmetry
+// functions, MLP kSgDesc-kSgHidden-kSgHidden-1.
+constexpr int kSgRadial = 24, kSgAngular = 8, kSgDesc = 32, kSgHidden = 64;
+constexpr int kSgDescParams = 6 + 2 * kSgAngular;  // rc eta_r mu0 dmu eta_a kind (p q) x 8
+constexpr int kSgWeights = kSgHidden * kSgDesc + kSgHidden + kSgHidden * kSgHidden + kSgHidden +
+                           kSgHidden + 1;
+struct SurrogateModel {
+  double desc[kSgDescParams] = {0};
+  double threshold = 0.0;
+  bool loaded = false;
+  uint16_t* wts = nullptr;  // bf16 weights (device)
+  float* vec = nullptr;     // biases, output weights, normalisation, gauge box
+  int* state = nullptr;     // fallback flag, step and fallback counts, gauge words
+  float* G = nullptr;       // descriptors, n x kSgDesc
+  float* dEdG = nullptr;    // dE/dG, n x kSgDesc
+  int cap = 0;
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -173,6 +191,8 @@ struct Ctx {
   int tab_index[3][kMaxTypes][kMaxTypes];  // kind -> (ta,tb) -> table id
   SwParams sw{};
   TersoffParams ters{};
+  bool ters_set = false;
+  SurrogateModel sg;
   double cut = 0;  // largest interaction cutoff
   // results of the last evaluation
   double epot = 0, virial = 0, ekin = 0;
@@ -269,6 +289,13 @@ void eam_forces(Ctx& c, bool energy);
 void pair_forces(Ctx& c, bool energy);
 void sw_forces(Ctx& c, bool energy);
 void tersoff_forces(Ctx& c, bool energy);
+// Tersoff passes that run only when *only_if != 0 at execution (device flag)
+void tersoff_forces_if(Ctx& c, bool energy, const int* only_if);
+void near_arrays(Ctx& c, int k);
+// learned surrogate (surrogate.hip)
+void surrogate_upload(Ctx& c, const double* desc, const double* norm, const float* wts);
+void surrogate_forces(Ctx& c, bool energy);
+void surrogate_free(Ctx& c);
 void integrate_kick_drift(Ctx& c, double dt);
 void integrate_kick(Ctx& c, double dt);
 // the POST_FORCE and POST_INTEGRATION modifies of MDSystem::FirstRun

@@ -1,0 +1,567 @@
+// Learned surrogate of the Tersoff potential (no counterpart in the
+// reference; it stands in for the bond-order force of
+// include/potential/tersoff.h:247-365 when a fitted model is loaded).
+//
+// Per atom i: a descriptor G_i of 24 radial and 8 angular symmetry functions
+// inside rc, an energy E_i = MLP(z_i) with z = (G - mean)/std and an MLP
+// 32-64-64-1 with SiLU activations, and forces from the chain rule.
+//   pass A  k_sg_desc    list scan -> near list (neighbours inside rc, with
+//                        displacements), G_i, the extrapolation gauge: how
+//                        far G_i lies outside the box of the training
+//                        descriptors (per-feature, in units of its width)
+//   decide  k_sg_decide  one thread: fall back to Tersoff for this step when
+//                        the largest gauge exceeds the threshold
+//   pass B  k_sg_mlp     forward and backward through the MLP on the matrix
+//                        cores: bf16 operands, FP32 accumulation
+//                        (v_mfma_f32_16x16x16_bf16), 16 atoms per wavefront
+//                        and instruction; writes E_i and dE/dG_i
+//   pass C  k_sg_centre  per centre i and near neighbour j: dE_i/dx_j (radial
+//                        and angular terms); the centre takes +sum_j of them
+//   pass D  k_sg_legs    each atom subtracts dE_i/dx_l stored by the centres
+//                        that have it as a neighbour
+// Every pass checks the fallback flag on the device and the Tersoff passes
+// the opposite, so a step needs no host round trip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "omdg_internal.h"
+
+namespace omdg {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int NR = kSgRadial, NA = kSgAngular, ND = kSgDesc, NH = kSgHidden;
+constexpr int kNear = 24;  // neighbours inside rc kept per atom (shares the SW arrays)
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct SgArgs {
+  const float4* q;
+  const uint32_t* nbr;
+  const int* nnbr;
+  float4* near;
+  int* near_n;
+  float4* gsum;
+  float* G;      // n x 32
+  float* dEdG;   // n x 32
+  float* f;
+  float* pot;
+  float* vir;
+  int* err;
+  unsigned int* gauge;  // largest gauge of the step (float bits)
+  int* state;           // [0] fallback flag, [1] steps, [2] fallback steps
+  unsigned int* gauge_max;
+  const uint16_t* wts;  // bf16 W1 (64x32), W2 (64x64), W2^T, W1^T (32x64)
+  const float* vec;     // b1, b2, w3 (64 each), b3, mean, inv_std, lo, inv_w (32 each)
+  int n;
+  float hx, hy, hz;
+  const int* rep;
+  const float4* rep_h;
+  float rc, rc2, eta_r, mu0, dmu, eta_a;
+  int kind;       // angular form: 1 Gaussian in cos, 0 2^(1-q) (1 + p cos)^q
+  float ap[NA], aq[NA];  // angular centre / width (kind 1) or lambda / exponent (kind 0)
+  int aqi[NA];           // the exponent as an integer (kind 0)
+  float threshold;
+  int energy;
+};
+
+constexpr int kW1 = 0, kW2 = NH * ND, kW2T = kW2 + NH * NH, kW1T = kW2T + NH * NH,
+              kWts = kW1T + ND * NH;
+constexpr int kB1 = 0, kB2 = NH, kW3 = 2 * NH, kB3 = 3 * NH, kMean = 3 * NH + 1,
+              kIstd = kMean + ND, kLo = kIstd + ND, kIw = kLo + ND, kVec = kIw + ND;
+
+__device__ inline float4 off_of(uint32_t code, const float4& h) {
+  const int c = (int)code;
+  return make_float4((float)(c % 3 - 1) * h.x, (float)((c / 3) % 3 - 1) * h.y,
+                     (float)(c / 9 - 1) * h.z, 0.f);
+}
+
+__device__ inline float ipow(float a, int k) {
+  float r = 1.f;
+  while (k > 0) {
+    if (k & 1) r *= a;
+    a *= a;
+    k >>= 1;
+  }
+  return r;
+}
+
+// angular function A_m(cos) and its derivative
+__device__ inline float ang(const SgArgs& a, int m, float cs, float& dA) {
+  if (a.kind == 1) {
+    const float t = cs - a.ap[m];
+    const float g = expf(-a.aq[m] * t * t);
+    dA = -2.f * a.aq[m] * t * g;
+    return g;
+  }
+  const float b = 1.f + a.ap[m] * cs;
+  const float p1 = ldexpf(ipow(b, a.aqi[m] - 1), 1 - a.aqi[m]);
+  dA = p1 * (float)a.aqi[m] * a.ap[m];
+  return p1 * b;
+}
+
+__device__ inline void cutoff(const SgArgs& a, float r, float& fc, float& dfc) {
+  const float t = 3.14159265358979f * r / a.rc;
+  fc = 0.5f * (cosf(t) + 1.f);
+  dfc = -0.5f * 3.14159265358979f / a.rc * sinf(t);
+}
+
+// ---- pass A -----------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_sg_desc(SgArgs a) {
+  const int l = xcd_block() * blockDim.x + threadIdx.x;
+  float gauge = 0.f;
+  if (l < a.n) {
+    const int n = a.n;
+    const float4 hh = a.rep ? a.rep_h[a.rep[l]] : make_float4(a.hx, a.hy, a.hz, 0.f);
+    const float4 ql = a.q[l];
+    float G[ND];
+#pragma unroll
+    for (int g = 0; g < ND; g++) G[g] = 0.f;
+    int nn = 0;
+    const int cnt = a.nnbr[l];
+    for (int k = 0; k < cnt; k++) {
+      const uint32_t e = a.nbr[nidx(l, k, n)];
+      const float4 qj = a.q[e & kSlotMask];
+      const float4 o = off_of(e >> kImgShift, hh);
+      const float dx = qj.x - ql.x + o.x, dy = qj.y - ql.y + o.y, dz = qj.z - ql.z + o.z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      if (!(r2 < a.rc2)) continue;
+      if (r2 < 1e-6f) {
+        atomicOr(a.err, ERR_OVERLAP);
+        continue;
+      }
+      if (nn < kNear) a.near[(size_t)nn * n + l] = make_float4(dx, dy, dz, __uint_as_float(e));
+      else atomicOr(a.err, ERR_NEAR);
+      nn++;
+    }
+    if (nn > kNear) nn = kNear;
+    for (int u = 0; u < nn; u++) {
+      const float4 du = a.near[(size_t)u * n + l];
+      const float ru = sqrtf(du.x * du.x + du.y * du.y + du.z * du.z);
+      float fu, dfu;
+      cutoff(a, ru, fu, dfu);
+#pragma unroll
+      for (int k = 0; k < NR; k++) {
+        const float t = ru - (a.mu0 + a.dmu * (float)k);
+        G[k] += expf(-a.eta_r * t * t) * fu;
+      }
+      for (int w = u + 1; w < nn; w++) {
+        const float4 dw = a.near[(size_t)w * n + l];
+        const float rw = sqrtf(dw.x * dw.x + dw.y * dw.y + dw.z * dw.z);
+        float fw, dfw;
+        cutoff(a, rw, fw, dfw);
+        const float cs = (du.x * dw.x + du.y * dw.y + du.z * dw.z) / (ru * rw);
+        const float rad = expf(-a.eta_a * (ru * ru + rw * rw)) * fu * fw;
+#pragma unroll
+        for (int m = 0; m < NA; m++) {
+          float dA;
+          G[NR + m] += ang(a, m, cs, dA) * rad;
+        }
+      }
+    }
+    a.near_n[l] = nn;
+    float4* gout = reinterpret_cast<float4*>(a.G + (size_t)l * ND);
+#pragma unroll
+    for (int g = 0; g < ND; g += 4) gout[g / 4] = make_float4(G[g], G[g + 1], G[g + 2], G[g + 3]);
+    // distance outside the box of the training descriptors
+#pragma unroll
+    for (int g = 0; g < ND; g++) {
+      const float lo = a.vec[kLo + g], iw = a.vec[kIw + g];
+      const float hi = lo + 1.f / iw;
+      gauge = fmaxf(gauge, fmaxf(lo - G[g], G[g] - hi) * iw);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) gauge = fmaxf(gauge, __shfl_xor(gauge, o, 64));
+  if ((threadIdx.x & 63) == 0 && gauge > 0.f) atomicMax(a.gauge, __float_as_uint(gauge));
+}
+
+__global__ void k_sg_decide(SgArgs a) {
+  const float g = __uint_as_float(*a.gauge);
+  const int fb = g > a.threshold ? 1 : 0;
+  a.state[0] = fb;
+  a.state[1] += 1;
+  a.state[2] += fb;
+  atomicMax(a.gauge_max, *a.gauge);
+  *a.gauge = 0u;
+}
+
+// ---- pass B: the MLP on the matrix cores ------------------------------------
+__device__ inline short to_bf16(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // nearest even
+  return (short)(u >> 16);
+}
+
+__device__ inline bf16x4 pack(float a, float b, float c, float d) {
+  bf16x4 r;
+  r[0] = to_bf16(a);
+  r[1] = to_bf16(b);
+  r[2] = to_bf16(c);
+  r[3] = to_bf16(d);
+  return r;
+}
+
+// A operand of rows [r0, r0+16) and columns [k0, k0+16) of a row-major bf16
+// matrix with ld columns: lane l holds row r0 + l%16, columns k0 + 4(l/16)..+3
+__device__ inline bf16x4 frag(const uint16_t* m, int ld, int r0, int k0, int lane) {
+  const uint2 v = *reinterpret_cast<const uint2*>(m + (r0 + (lane & 15)) * ld + k0 + 4 * (lane >> 4));
+  bf16x4 r;
+  r[0] = (short)(v.x & 0xFFFF);
+  r[1] = (short)(v.x >> 16);
+  r[2] = (short)(v.y & 0xFFFF);
+  r[3] = (short)(v.y >> 16);
+  return r;
+}
+
+__device__ inline float sigm(float x) { return 1.f / (1.f + expf(-x)); }
+
+__global__ void __launch_bounds__(kBlock) k_sg_mlp(SgArgs a) {
+  if (a.state[0]) return;  // this step falls back to Tersoff
+  __shared__ __align__(16) uint16_t w[kWts];
+  __shared__ float v[kVec];
+  for (int i = threadIdx.x; i < kWts / 2; i += blockDim.x)
+    reinterpret_cast<uint32_t*>(w)[i] = reinterpret_cast<const uint32_t*>(a.wts)[i];
+  for (int i = threadIdx.x; i < kVec; i += blockDim.x) v[i] = a.vec[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int col = lane & 15, grp = lane >> 4;  // D: row 4 grp + item, column col
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nwaves = (gridDim.x * blockDim.x) >> 6;
+  const int ngroups = (a.n + 15) / 16;
+  for (int gi = wave; gi < ngroups; gi += nwaves) {
+    const int atom = gi * 16 + col;
+    const bool live = atom < a.n;
+    // B operand of layer 1: z features 16 kb + 4 grp + item of this atom
+    bf16x4 bz[2];
+#pragma unroll
+    for (int kb = 0; kb < 2; kb++) {
+      const int g0 = kb * 16 + 4 * grp;
+      float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (live) gv = *reinterpret_cast<const float4*>(a.G + (size_t)atom * ND + g0);
+      bz[kb] = pack((gv.x - v[kMean + g0]) * v[kIstd + g0],
+                    (gv.y - v[kMean + g0 + 1]) * v[kIstd + g0 + 1],
+                    (gv.z - v[kMean + g0 + 2]) * v[kIstd + g0 + 2],
+                    (gv.w - v[kMean + g0 + 3]) * v[kIstd + g0 + 3]);
+    }
+    // layer 1
+    f32x4 a1[4];
+    bf16x4 bh1[4];
+#pragma unroll
+    for (int ob = 0; ob < 4; ob++) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < 2; kb++)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag(w + kW1, ND, ob * 16, kb * 16, lane),
+                                                        bz[kb], acc, 0, 0, 0);
+      float h[4];
+#pragma unroll
+      for (int it = 0; it < 4; it++) {
+        const float z = acc[it] + v[kB1 + ob * 16 + 4 * grp + it];
+        a1[ob][it] = z;
+        h[it] = z * sigm(z);
+      }
+      bh1[ob] = pack(h[0], h[1], h[2], h[3]);
+    }
+    // layer 2, energy, delta of layer 2
+    float e = 0.f;
+    bf16x4 bd2[4];
+#pragma unroll
+    for (int ob = 0; ob < 4; ob++) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < 4; kb++)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag(w + kW2, NH, ob * 16, kb * 16, lane),
+                                                        bh1[kb], acc, 0, 0, 0);
+      float d[4];
+#pragma unroll
+      for (int it = 0; it < 4; it++) {
+        const int f = ob * 16 + 4 * grp + it;
+        const float z = acc[it] + v[kB2 + f];
+        const float s = sigm(z);
+        e += z * s * v[kW3 + f];
+        d[it] = v[kW3 + f] * s * (1.f + z * (1.f - s));
+      }
+      bd2[ob] = pack(d[0], d[1], d[2], d[3]);
+    }
+    e += __shfl_xor(e, 16, 64);
+    e += __shfl_xor(e, 32, 64);
+    if (a.energy && live && grp == 0) {
+      a.pot[atom] = e + v[kB3];
+      a.vir[atom] = 0.f;
+    }
+    // back through layer 2: delta of layer 1
+    bf16x4 bd1[4];
+#pragma unroll
+    for (int ob = 0; ob < 4; ob++) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < 4; kb++)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag(w + kW2T, NH, ob * 16, kb * 16, lane),
+                                                        bd2[kb], acc, 0, 0, 0);
+      float d[4];
+#pragma unroll
+      for (int it = 0; it < 4; it++) {
+        const float z = a1[ob][it];
+        const float s = sigm(z);
+        d[it] = acc[it] * s * (1.f + z * (1.f - s));
+      }
+      bd1[ob] = pack(d[0], d[1], d[2], d[3]);
+    }
+    // back through layer 1: dE/dz, then dE/dG = dE/dz / std
+#pragma unroll
+    for (int gb = 0; gb < 2; gb++) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < 4; kb++)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag(w + kW1T, NH, gb * 16, kb * 16, lane),
+                                                        bd1[kb], acc, 0, 0, 0);
+      const int g0 = gb * 16 + 4 * grp;
+      if (live)
+        *reinterpret_cast<float4*>(a.dEdG + (size_t)atom * ND + g0) =
+            make_float4(acc[0] * v[kIstd + g0], acc[1] * v[kIstd + g0 + 1],
+                        acc[2] * v[kIstd + g0 + 2], acc[3] * v[kIstd + g0 + 3]);
+    }
+  }
+}
+
+// ---- pass C -----------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_sg_centre(SgArgs a) {
+  if (a.state[0]) return;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int n = a.n;
+  const int nn = a.near_n[i];
+  float dg[ND];
+  const float4* gi = reinterpret_cast<const float4*>(a.dEdG + (size_t)i * ND);
+#pragma unroll
+  for (int g = 0; g < ND; g += 4) {
+    const float4 t = gi[g / 4];
+    dg[g] = t.x;
+    dg[g + 1] = t.y;
+    dg[g + 2] = t.z;
+    dg[g + 3] = t.w;
+  }
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+  for (int u = 0; u < nn; u++) {
+    const float4 du = a.near[(size_t)u * n + i];
+    const float ru = sqrtf(du.x * du.x + du.y * du.y + du.z * du.z);
+    const float iru = 1.f / ru;
+    float fu, dfu;
+    cutoff(a, ru, fu, dfu);
+    // radial: sum_k dE/dG_k phi_k'(r) / r
+    float coef = 0.f;
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+      const float t = ru - (a.mu0 + a.dmu * (float)k);
+      const float g = expf(-a.eta_r * t * t);
+      coef += dg[k] * g * (-2.f * a.eta_r * t * fu + dfu);
+    }
+    coef *= iru;
+    float gx = coef * du.x, gy = coef * du.y, gz = coef * du.z;
+    // angular: d T(u, w) / d x_u for every other leg w
+    for (int w = 0; w < nn; w++) {
+      if (w == u) continue;
+      const float4 dw = a.near[(size_t)w * n + i];
+      const float rw = sqrtf(dw.x * dw.x + dw.y * dw.y + dw.z * dw.z);
+      float fw, dfw;
+      cutoff(a, rw, fw, dfw);
+      const float irw = 1.f / rw;
+      const float cs = (du.x * dw.x + du.y * dw.y + du.z * dw.z) * iru * irw;
+      const float ea = expf(-a.eta_a * (ru * ru + rw * rw));
+      const float rad = ea * fu * fw;
+      const float drad = -2.f * a.eta_a * rad + ea * dfu * fw * iru;  // times d_u
+      float sc = 0.f, sr = 0.f;  // coefficients of d cos/d x_u and of d_u
+#pragma unroll
+      for (int m = 0; m < NA; m++) {
+        float dA;
+        const float A = ang(a, m, cs, dA);
+        sc += dg[NR + m] * dA * rad;
+        sr += dg[NR + m] * A * drad;
+      }
+      // d cos / d x_u = d_w / (r_u r_w) - cos d_u / r_u^2
+      const float cw = sc * iru * irw, cu = sr - sc * cs * iru * iru;
+      gx += cw * dw.x + cu * du.x;
+      gy += cw * dw.y + cu * du.y;
+      gz += cw * dw.z + cu * du.z;
+    }
+    a.gsum[(size_t)u * n + i] = make_float4(gx, gy, gz, 0.f);
+    fx += gx;
+    fy += gy;
+    fz += gz;
+  }
+  a.f[i] = fx;
+  a.f[(size_t)n + i] = fy;
+  a.f[2 * (size_t)n + i] = fz;
+}
+
+// ---- pass D -----------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_sg_legs(SgArgs a) {
+  if (a.state[0]) return;
+  const int l = xcd_block() * blockDim.x + threadIdx.x;
+  if (l >= a.n) return;
+  const int n = a.n;
+  const int nn = a.near_n[l];
+  float fx = a.f[l], fy = a.f[(size_t)n + l], fz = a.f[2 * (size_t)n + l];
+  for (int u = 0; u < nn; u++) {
+    const float4 du = a.near[(size_t)u * n + l];
+    const int c = (int)(__float_as_uint(du.w) & kSlotMask);
+    const int nc = a.near_n[c];
+    for (int s = 0; s < nc; s++) {
+      const float4 dk = a.near[(size_t)s * n + c];  // x_k - x_c
+      if ((int)(__float_as_uint(dk.w) & kSlotMask) != l) continue;
+      const float ex = dk.x + du.x, ey = dk.y + du.y, ez = dk.z + du.z;
+      if (ex * ex + ey * ey + ez * ez >= 1e-4f) continue;  // another image of l
+      const float4 g = a.gsum[(size_t)s * n + c];
+      fx -= g.x;
+      fy -= g.y;
+      fz -= g.z;
+      break;
+    }
+  }
+  a.f[l] = fx;
+  a.f[(size_t)n + l] = fy;
+  a.f[2 * (size_t)n + l] = fz;
+}
+
+}  // namespace
+
+// Upload the model: weights to bf16 (row-major and transposed copies),
+// vectors in FP32.
+void surrogate_upload(Ctx& c, const double* desc, const double* norm, const float* wts) {
+  SurrogateModel& m = c.sg;
+  std::memcpy(m.desc, desc, sizeof(m.desc));
+  std::vector<uint16_t> w(kWts);
+  auto bf = [](float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+  };
+  const float* W1 = wts;
+  const float* b1 = W1 + NH * ND;
+  const float* W2 = b1 + NH;
+  const float* b2 = W2 + NH * NH;
+  const float* w3 = b2 + NH;
+  const float* b3 = w3 + NH;
+  for (int o = 0; o < NH; o++)
+    for (int k = 0; k < ND; k++) {
+      w[kW1 + o * ND + k] = bf(W1[o * ND + k]);
+      w[kW1T + k * NH + o] = bf(W1[o * ND + k]);
+    }
+  for (int o = 0; o < NH; o++)
+    for (int k = 0; k < NH; k++) {
+      w[kW2 + o * NH + k] = bf(W2[o * NH + k]);
+      w[kW2T + k * NH + o] = bf(W2[o * NH + k]);
+    }
+  std::vector<float> v(kVec);
+  for (int o = 0; o < NH; o++) {
+    v[kB1 + o] = b1[o];
+    v[kB2 + o] = b2[o];
+    v[kW3 + o] = w3[o];
+  }
+  v[kB3] = b3[0];
+  for (int g = 0; g < ND; g++) {
+    v[kMean + g] = (float)norm[g];
+    v[kIstd + g] = (float)(1.0 / norm[ND + g]);
+    v[kLo + g] = (float)norm[2 * ND + g];
+    v[kIw + g] = (float)(1.0 / (norm[3 * ND + g] - norm[2 * ND + g]));
+  }
+  if (!m.wts) OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&m.wts), sizeof(uint16_t) * kWts));
+  if (!m.vec) OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&m.vec), sizeof(float) * kVec));
+  if (!m.state) OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&m.state), sizeof(int) * 8));
+  OMDG_HIP(hipMemcpy(m.wts, w.data(), sizeof(uint16_t) * kWts, hipMemcpyHostToDevice));
+  OMDG_HIP(hipMemcpy(m.vec, v.data(), sizeof(float) * kVec, hipMemcpyHostToDevice));
+  OMDG_HIP(hipMemset(m.state, 0, sizeof(int) * 8));
+  m.loaded = true;
+}
+
+void surrogate_free(Ctx& c) {
+  SurrogateModel& m = c.sg;
+  if (m.wts) (void)hipFree(m.wts);
+  if (m.vec) (void)hipFree(m.vec);
+  if (m.state) (void)hipFree(m.state);
+  if (m.G) (void)hipFree(m.G);
+  if (m.dEdG) (void)hipFree(m.dEdG);
+  m = SurrogateModel{};
+}
+
+// The surrogate step: passes A-D; the Tersoff passes run behind the same
+// device flag for the steps that fall back.
+void surrogate_forces(Ctx& c, bool energy) {
+  SurrogateModel& m = c.sg;
+  if (!m.loaded) throw ApiFail(OMDG_EINVAL, "no surrogate model (omdg_set_surrogate)");
+  const int n = c.n;
+  if (m.cap < n) {
+    if (m.G) (void)hipFree(m.G);
+    if (m.dEdG) (void)hipFree(m.dEdG);
+    m.G = m.dEdG = nullptr;
+    OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&m.G), sizeof(float) * ND * (size_t)n));
+    OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&m.dEdG), sizeof(float) * ND * (size_t)n));
+    m.cap = n;
+  }
+  near_arrays(c, kNear);
+  SgArgs a{};
+  a.q = c.q;
+  a.nbr = c.nbr;
+  a.nnbr = c.nnbr;
+  a.near = c.near;
+  a.near_n = c.near_n;
+  a.gsum = c.gsum;
+  a.G = m.G;
+  a.dEdG = m.dEdG;
+  a.f = c.f32;
+  a.pot = c.pot32;
+  a.vir = c.vir32;
+  a.err = c.dstat + ST_ERR;
+  a.gauge = reinterpret_cast<unsigned int*>(m.state + 4);
+  a.gauge_max = reinterpret_cast<unsigned int*>(m.state + 5);
+  a.state = m.state;
+  a.wts = m.wts;
+  a.vec = m.vec;
+  a.n = n;
+  a.hx = (float)c.grid.h[0];
+  a.hy = (float)c.grid.h[1];
+  a.hz = (float)c.grid.h[2];
+  if (c.nrep > 0) {
+    a.rep = c.rep;
+    a.rep_h = c.rep_h;
+  }
+  a.rc = (float)m.desc[0];
+  a.rc2 = a.rc * a.rc;
+  a.eta_r = (float)m.desc[1];
+  a.mu0 = (float)m.desc[2];
+  a.dmu = (float)m.desc[3];
+  a.eta_a = (float)m.desc[4];
+  a.kind = (int)m.desc[5];
+  for (int k = 0; k < NA; k++) {
+    a.ap[k] = (float)m.desc[6 + 2 * k];
+    a.aq[k] = (float)m.desc[7 + 2 * k];
+    a.aqi[k] = (int)m.desc[7 + 2 * k];
+  }
+  a.threshold = (float)m.threshold;
+  a.energy = energy ? 1 : 0;
+  const int nb = grid_blocks(n, kBlock);
+  {
+    TimerScope ts(c, TG_PASS1);
+    k_sg_desc<<<nb, kBlock, 0, c.stream>>>(a);
+    k_sg_decide<<<1, 1, 0, c.stream>>>(a);
+  }
+  {
+    TimerScope ts(c, TG_FORCE);
+    const int groups = (n + 15) / 16;
+    k_sg_mlp<<<std::min(2048, std::max(1, (groups + 3) / 4)), kBlock, 0, c.stream>>>(a);
+    k_sg_centre<<<nb, kBlock, 0, c.stream>>>(a);
+    k_sg_legs<<<nb, kBlock, 0, c.stream>>>(a);
+  }
+  OMDG_HIP(hipGetLastError());
+  // the steps whose descriptors left the training box: Tersoff, behind the
+  // same flag
+  tersoff_forces_if(c, energy, m.state);
+}
+
+}  // namespace omdg

@@ -48,6 +48,7 @@ struct TBArgs {
   float4* near;   // SW: near list, kMaxNear slot rows of n (x_j - x_l, entry bits)
   float4* gsum;   // SW: per near entry, force on that leg from the centre's triplets
   int* near_n;    // SW: near-list length per atom
+  const int* only_if;  // Tersoff passes run only when *only_if != 0 (if set)
   int* err;
   int n;
   float hx, hy, hz;
@@ -413,6 +414,7 @@ __device__ inline float ters_zeta_term(const TersoffParams& p, float l3m, const 
 
 template <bool ENERGY>
 __global__ void __launch_bounds__(kBlock) k_tersoff_bond(TBArgs a) {
+  if (a.only_if && !*a.only_if) return;  // uniform: a step of the surrogate
   __shared__ float4 offs[27];
   load_offsets(offs, a.hx, a.hy, a.hz);
   __syncthreads();
@@ -491,6 +493,7 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_bond(TBArgs a) {
 }
 
 __global__ void __launch_bounds__(kBlock) k_tersoff_force(TBArgs a) {
+  if (a.only_if && !*a.only_if) return;
   __shared__ float4 offs[27];
   load_offsets(offs, a.hx, a.hy, a.hz);
   __syncthreads();
@@ -633,6 +636,25 @@ TBArgs base_args(Ctx& c) {
 
 }  // namespace
 
+// Near-list storage shared by the SW and surrogate passes: k float4
+// entries per atom (slot-major), a per-entry float4 of leg results, and the
+// per-atom entry count.
+void near_arrays(Ctx& c, int k) {
+  if ((size_t)c.n * k <= c.cap_near) return;
+  for (float4** pp : {&c.near, &c.gsum}) {
+    if (*pp) (void)hipFree(*pp);
+    *pp = nullptr;
+  }
+  if (c.near_n) (void)hipFree(c.near_n);
+  c.near_n = nullptr;
+  c.cap_near = 0;
+  const size_t bytes = sizeof(float4) * k * (size_t)c.n;
+  OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.near), bytes));
+  OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.gsum), bytes));
+  OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.near_n), sizeof(int) * (size_t)c.n));
+  c.cap_near = (size_t)c.n * k;
+}
+
 void sw_forces(Ctx& c, bool energy) {
   TBArgs a = base_args(c);
   const SwParams& p = c.sw;
@@ -683,20 +705,7 @@ void sw_forces(Ctx& c, bool energy) {
   const double pd = p.p, qd = p.q;
   a.pi = (pd == std::floor(pd) && pd >= 0 && pd <= 16) ? (int)pd : -1;
   a.qi = (qd == std::floor(qd) && qd >= 0 && qd <= 16) ? (int)qd : -1;
-  if ((size_t)c.n * kMaxNear > c.cap_near) {
-    for (float4** pp : {&c.near, &c.gsum}) {
-      if (*pp) (void)hipFree(*pp);
-      *pp = nullptr;
-    }
-    if (c.near_n) (void)hipFree(c.near_n);
-    c.near_n = nullptr;
-    c.cap_near = 0;
-    const size_t bytes = sizeof(float4) * kMaxNear * (size_t)c.n;
-    OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.near), bytes));
-    OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.gsum), bytes));
-    OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.near_n), sizeof(int) * (size_t)c.n));
-    c.cap_near = (size_t)c.n * kMaxNear;
-  }
+  near_arrays(c, kMaxNear);
   a.near = c.near;
   a.gsum = c.gsum;
   a.near_n = c.near_n;
@@ -719,8 +728,9 @@ void sw_forces(Ctx& c, bool energy) {
   OMDG_HIP(hipGetLastError());
 }
 
-void tersoff_forces(Ctx& c, bool energy) {
+void tersoff_forces_if(Ctx& c, bool energy, const int* only_if) {
   TBArgs a = base_args(c);
+  a.only_if = only_if;
   a.ts = c.ters;
   a.rc = c.ters.R + c.ters.D;
   a.rc2 = a.rc * a.rc;
@@ -747,5 +757,7 @@ void tersoff_forces(Ctx& c, bool energy) {
   }
   OMDG_HIP(hipGetLastError());
 }
+
+void tersoff_forces(Ctx& c, bool energy) { tersoff_forces_if(c, energy, nullptr); }
 
 }  // namespace omdg

@@ -1,0 +1,182 @@
+"""FP64 CPU restatement of the learned Tersoff surrogate (the oracle of the
+surrogate path).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, always as the checker.
+
+The surrogate has no counterpart in the reference; it replaces the Tersoff
+bond-order force (include/potential/tersoff.h:247-365) by a per-atom energy
+E_i = MLP(G_i) of a symmetry-function descriptor G_i, so the restatement
+follows the model's own definition:
+
+  fc(r)    = (cos(pi r / rc) + 1) / 2 for r < rc
+  radial   G_k = sum_j exp(-eta_r (r_ij - mu_k)^2) fc(r_ij),  mu_k = mu0 + k dmu
+  angular  G_m = sum_{j<k} A_m(cos t_jik) exp(-eta_a (r_ij^2 + r_ik^2))
+                           fc(r_ij) fc(r_ik)
+           with A_m(c) = exp(-q_m (c - p_m)^2)                    (kind 1)
+             or A_m(c) = 2^(1-q_m) (1 + p_m c)^q_m, q_m integer  (kind 0)
+  z        = (G - mean) / std
+  E_i      = w3 . silu(W2 silu(W1 z + b1) + b2) + b3
+  F_k      = - sum_i dE_i/dx_k   (chain rule through z and G)
+
+With bf16=True the products use the BF16 operands the device kernel feeds
+its matrix cores (weights, z, hidden activations and back-propagated
+deltas rounded to BF16, sums in FP64), so the two agree to summation order.
+"""
+import numpy as np
+
+from oracle import oracle as O
+
+NR, NA, NH = 24, 8, 64
+ND = NR + NA
+
+
+def default_desc():
+    """rc, eta_r, mu0, dmu, eta_a, the angular kind and its 8 (p, q) pairs:
+    Gaussians in cos t centred at -1 ... 1."""
+    pq = [(-1.0 + 2.0 * m / 7.0, 24.0) for m in range(NA)]
+    return dict(rc=3.0, eta_r=50.0, mu0=1.5, dmu=1.5 / 23.0, eta_a=0.0, kind=1, pq=pq)
+
+
+def desc_vector(d):
+    """The 22 doubles of omdg_set_surrogate's descriptor block."""
+    out = [d["rc"], d["eta_r"], d["mu0"], d["dmu"], d["eta_a"], float(d["kind"])]
+    for p, q in d["pq"]:
+        out += [p, q]
+    return out
+
+
+def _ang(d, m, cos):
+    """A_m(cos) and dA_m/dcos."""
+    p, q = d["pq"][m]
+    if d["kind"] == 1:
+        g = np.exp(-q * (cos - p) ** 2)
+        return g, -2.0 * q * (cos - p) * g
+    a = 1.0 + p * cos
+    return 2.0 ** (1.0 - q) * a ** q, 2.0 ** (1.0 - q) * q * a ** (q - 1.0) * p
+
+
+def bf16(a):
+    """Round to BF16 (nearest even), returned as float64."""
+    a32 = np.ascontiguousarray(a, np.float32)
+    u = a32.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _fc(r, rc):
+    c = 0.5 * (np.cos(np.pi * r / rc) + 1.0)
+    dc = -0.5 * np.pi / rc * np.sin(np.pi * r / rc)
+    inside = r < rc
+    return np.where(inside, c, 0.0), np.where(inside, dc, 0.0)
+
+
+def pairs(x, lo, L, pbc, rc, maxn=64):
+    """Every (i, j, image) with r_ij < rc: arrays i, j and d = x_j(image) - x_i."""
+    xw = O.wrap(x, lo, L, pbc)
+    cnt, code = O.neighbors(xw, lo, L, pbc, rc, maxn=maxn)
+    ii, jj, dd = [], [], []
+    for i in range(len(xw)):
+        for c in code[i, :cnt[i]]:
+            j, img = divmod(int(c), 27)
+            s = np.array([img % 3 - 1, img // 3 % 3 - 1, img // 9 - 1], float)
+            d = xw[j] + s * L - xw[i]
+            if d @ d < rc * rc:
+                ii.append(i)
+                jj.append(j)
+                dd.append(d)
+    return np.array(ii, int), np.array(jj, int), np.array(dd).reshape(-1, 3), xw
+
+
+def descriptors(n, ii, dd, d):
+    """G (n, 32) and, per pair, the pieces needed by the force pass."""
+    rc = d["rc"]
+    r = np.sqrt((dd * dd).sum(1))
+    fc, dfc = _fc(r, rc)
+    mu = d["mu0"] + d["dmu"] * np.arange(NR)
+    g = np.exp(-d["eta_r"] * (r[:, None] - mu[None, :]) ** 2)
+    G = np.zeros((n, ND))
+    np.add.at(G[:, :NR], ii, g * fc[:, None])
+    # triplets: ordered pairs (u, w), u != w, of the pairs of one centre
+    tri = []
+    for i in range(n):
+        idx = np.nonzero(ii == i)[0]
+        for u in idx:
+            for w in idx:
+                if u != w:
+                    tri.append((i, u, w))
+    tri = np.array(tri, int).reshape(-1, 3)
+    if len(tri):
+        c, u, w = tri[:, 0], tri[:, 1], tri[:, 2]
+        cos = (dd[u] * dd[w]).sum(1) / (r[u] * r[w])
+        rad = np.exp(-d["eta_a"] * (r[u] ** 2 + r[w] ** 2)) * fc[u] * fc[w]
+        for m in range(NA):
+            t = _ang(d, m, cos)[0] * rad
+            np.add.at(G[:, NR + m], c, 0.5 * t)  # each unordered triplet twice
+    return G, r, fc, dfc, tri
+
+
+def mlp(G, model, bf=True):
+    """Per-atom energies and dE/dG (n, 32)."""
+    q = bf16 if bf else (lambda a: np.asarray(a, np.float64))
+    W1, b1, W2, b2, w3, b3 = (model[k] for k in ("W1", "b1", "W2", "b2", "w3", "b3"))
+    W1, W2, w3 = q(W1), q(W2), np.asarray(w3, np.float64)
+    z = q((G - model["mean"]) / model["std"])
+    a1 = z @ W1.T + b1
+    s1 = 1.0 / (1.0 + np.exp(-a1))
+    h1 = a1 * s1
+    a2 = q(h1) @ W2.T + b2
+    s2 = 1.0 / (1.0 + np.exp(-a2))
+    h2 = a2 * s2
+    E = h2 @ w3 + b3
+    # back-propagation: silu'(a) = s (1 + a (1 - s))
+    d2 = w3[None, :] * s2 * (1.0 + a2 * (1.0 - s2))
+    d1 = (q(d2) @ W2) * s1 * (1.0 + a1 * (1.0 - s1))
+    dz = q(d1) @ W1
+    return E, dz / model["std"]
+
+
+def forces(n, ii, dd, r, fc, dfc, tri, dEdG, d):
+    """dE_i/dx_j for every pair (i, j) from dE/dG and the descriptor
+    derivatives (the force on j is minus it, the centre gets plus it)."""
+    mu = d["mu0"] + d["dmu"] * np.arange(NR)
+    # radial: d/dx_j of phi(r_ij) = phi'(r) d/r; on the centre the opposite
+    g = np.exp(-d["eta_r"] * (r[:, None] - mu[None, :]) ** 2)
+    dphi = g * (-2.0 * d["eta_r"] * (r[:, None] - mu[None, :]) * fc[:, None] + dfc[:, None])
+    coef = (dEdG[ii, :NR] * dphi).sum(1) / r
+    gvec = coef[:, None] * dd  # dE_i/dx_j per pair
+    # angular
+    if len(tri):
+        c, u, w = tri[:, 0], tri[:, 1], tri[:, 2]
+        ru, rw = r[u], r[w]
+        cos = (dd[u] * dd[w]).sum(1) / (ru * rw)
+        ea = np.exp(-d["eta_a"] * (ru ** 2 + rw ** 2))
+        rad = ea * fc[u] * fc[w]
+        dcos = dd[w] / (ru * rw)[:, None] - (cos / ru ** 2)[:, None] * dd[u]
+        drad = (-2.0 * d["eta_a"] * rad + ea * dfc[u] * fc[w] / ru)[:, None] * dd[u]
+        acc = np.zeros((len(tri), 3))
+        for m in range(NA):
+            A, dA = _ang(d, m, cos)
+            dt = (dA * rad)[:, None] * dcos + A[:, None] * drad
+            acc += dEdG[c, NR + m][:, None] * dt
+        # each unordered triplet {u, w} appears as (u, w) and (w, u): its
+        # derivative with respect to each leg is counted once
+        np.add.at(gvec, u, acc)
+    # force on j: -dE_i/dx_j; on the centre: +sum_j dE_i/dx_j
+    return gvec
+
+
+def energy_forces(x, lo, hi, pbc, model, d=None, bf=True):
+    """Total energy, per-atom energies and forces of the surrogate."""
+    d = default_desc() if d is None else d
+    lo = np.asarray(lo, float)
+    L = np.asarray(hi, float) - lo
+    n = len(x)
+    ii, jj, dd, _ = pairs(x, lo, L, pbc, d["rc"])
+    G, r, fc, dfc, tri = descriptors(n, ii, dd, d)
+    E, dEdG = mlp(G, model, bf)
+    gvec = forces(n, ii, dd, r, fc, dfc, tri, dEdG, d)
+    F = np.zeros((n, 3))
+    np.add.at(F, jj, -gvec)
+    np.add.at(F, ii, gvec)
+    return E.sum(), E, F, G

@@ -1,0 +1,124 @@
+"""GPU parity of the learned-surrogate path: descriptor, BF16 matrix-core MLP
+(forward and backward) and chain-rule forces against the FP64 oracle with the
+same BF16 operands; the device-side fallback to Tersoff; the trained model's
+accuracy against Tersoff."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import objectmd_amd as om
+from objectmd_amd import lattice, tables
+from objectmd_amd import surrogate as S
+from oracle import surrogate as OS
+from tests.common import rel_rms, tersoff_params, wrap_np
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _random_model(seed=3, wide_box=True):
+    rng = np.random.default_rng(seed)
+    m = dict(W1=rng.normal(0, 0.3, (64, 32)), b1=rng.normal(0, 0.1, 64),
+             W2=rng.normal(0, 0.2, (64, 64)), b2=rng.normal(0, 0.1, 64),
+             w3=rng.normal(0, 0.2, 64), b3=np.float64(-4.0), mean=np.full(32, 0.3),
+             std=np.full(32, 0.5))
+    m["lo"] = np.full(32, -1e3) if wide_box else np.full(32, -1e-3)
+    m["hi"] = np.full(32, 1e3) if wide_box else np.full(32, 1e-3)
+    return m
+
+
+def _slab(cells, seed, disp=0.12, a=5.431):
+    pos, (lo, hi) = lattice.diamond100(cells, cells, cells, a)
+    rng = np.random.default_rng(seed)
+    x = wrap_np(pos + rng.normal(0, disp, pos.shape), lo, hi - lo, 3)
+    return x, lo, hi
+
+
+def _engine(x, lo, hi, model, threshold=0.25):
+    eng = om.Engine()
+    eng.set_box(lo, hi, 3)
+    eng.upload_atoms(x, mass=[tables.mass_sim(tables.MASS_SI)])
+    eng.set_neighbor(0.4, 10, 0)
+    S.configure(eng, model, S.default_desc(), tersoff_params(2.7, 0.15), threshold)
+    return eng
+
+
+@pytest.mark.parametrize("cells,seed", [(3, 1), (5, 2)])
+def test_surrogate_vs_oracle(cells, seed):
+    x, lo, hi = _slab(cells, seed)
+    say(f"[surrogate] n={len(x)}")
+    m = _random_model()
+    eng = _engine(x, lo, hi, m)
+    ep, _ = eng.compute()
+    f = eng.download(om.F)
+    pot = eng.download(om.POT)
+    E, Ei, fo, G = OS.energy_forces(x, lo, hi, 3, m, OS.default_desc(), bf=True)
+    err_f = rel_rms(f, fo)
+    err_e = np.abs(pot - Ei).max()
+    say(f"[surrogate] force relRMS={err_f:.3e} max|dE_i|={err_e:.3e} Ep={ep:.6f} oracle={E:.6f}")
+    steps, fb, gmax = eng.surrogate_stats()
+    assert steps == 1 and fb == 0
+    assert err_f < 2e-3
+    assert err_e < 2e-3
+    assert ep == pytest.approx(E, rel=1e-5)
+    eng.close()
+
+
+def test_fallback_to_tersoff():
+    """Descriptors outside the (tiny) training box: the step is Tersoff's."""
+    x, lo, hi = _slab(4, 5)
+    eng = _engine(x, lo, hi, _random_model(wide_box=False))
+    eng.compute()
+    f_sg = eng.download(om.F)
+    steps, fb, gmax = eng.surrogate_stats()
+    say(f"[surrogate] fallback steps={steps} fb={fb} gauge={gmax:.3e}")
+    assert steps == 1 and fb == 1 and gmax > 0.25
+    ref = om.Engine()
+    ref.set_box(lo, hi, 3)
+    ref.upload_atoms(x, mass=[tables.mass_sim(tables.MASS_SI)])
+    ref.set_neighbor(0.4, 10, 0)
+    ref.set_tersoff(tersoff_params(2.7, 0.15))
+    ref.set_potential(om.POT_TERSOFF)
+    ref.compute()
+    assert np.array_equal(f_sg, ref.download(om.F))
+    eng.close()
+    ref.close()
+
+
+@pytest.mark.skipif(not os.path.exists(S.MODEL_PATH), reason="no trained surrogate model")
+def test_trained_model_against_tersoff():
+    """A 1,000 K slab the model has not seen: forces within 60 meV/A RMSE of
+    Tersoff's, energies per atom within 20 meV, no fallback."""
+    model, desc, meta = S.load()
+    x, lo, hi = _slab(8, 99, disp=0.0)
+    m = tables.mass_sim(tables.MASS_SI)
+    ter = om.Engine()
+    ter.set_box(lo, hi, 3)
+    ter.upload_atoms(x, lattice.maxwell_boltzmann(len(x), m, 2000.0, 5), mass=[m])
+    ter.set_neighbor(0.5, 10, 0)
+    ter.set_tersoff(tersoff_params(2.7, 0.15))
+    ter.set_potential(om.POT_TERSOFF)
+    ter.compute()
+    ter.zero_forces()
+    ter.step(1000, 0.001)
+    xs = ter.download(om.X)
+    ep_t, _ = ter.compute()
+    ft = ter.download(om.F)
+    sg = _engine(xs, lo, hi, model)
+    ep_s, _ = sg.compute()
+    fs = sg.download(om.F)
+    rmse = float(np.sqrt(((fs - ft) ** 2).mean()))
+    de = abs(ep_s - ep_t) / len(xs)
+    steps, fb, gmax = sg.surrogate_stats()
+    say(f"[surrogate] trained: force RMSE {rmse * 1e3:.1f} meV/A, |dE|/N {de * 1e3:.2f} meV, "
+        f"gauge {gmax:.3f}, hold-out {float(meta['force_rmse_holdout']) * 1e3:.1f} meV/A")
+    assert fb == 0
+    assert rmse < 0.06
+    assert de < 0.02
+    ter.close()
+    sg.close()
