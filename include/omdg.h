@@ -187,7 +187,7 @@ int omdg_batch_run(omdg_t ctx, int nsteps, double* time, double* ekin,
  * enable = 0 stops recording and returns, per group, the summed milliseconds
  * and the launch count (groups: 0 list build, 1 density/first pass, 2 force
  * pass, 3 kick+drift, 4 kick, 5 reductions, 6 replays of captured replica
- * steps).  max = size of the arrays. */
+ * steps, 7 the surrogate's matrix-core pass).  max = size of the arrays. */
 int omdg_timing(omdg_t ctx, int enable, int max, double* ms, int* count);
 
 #ifdef __cplusplus

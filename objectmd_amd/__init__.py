@@ -28,7 +28,7 @@ ABI_SYMBOLS = (
     "omdg_compute", "omdg_zero_forces", "omdg_step", "omdg_set_modifies",
     "omdg_set_replicas", "omdg_batch_run", "omdg_timing",
 )
-TIMING_GROUPS = ("build", "pass1", "force", "kick_drift", "kick", "reduce", "graph")
+TIMING_GROUPS = ("build", "pass1", "force", "kick_drift", "kick", "reduce", "graph", "mlp")
 
 _lib = None
 

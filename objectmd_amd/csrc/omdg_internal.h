@@ -144,8 +144,8 @@ struct Ctx {
   float* fp = nullptr;  // F'(rho)
   float* emb = nullptr; // F(rho)
   float* dmax = nullptr;    // per-block maxima of the drift (2 per block)
-  float* bond_b = nullptr;  // Tersoff per-slot bond order (ELL like nbr)
-  float* bond_p = nullptr;  // Tersoff per-slot zeta prefactor
+  float* bond_b = nullptr;  // Tersoff bond order per near slot (slot-major like near)
+  float* bond_p = nullptr;  // Tersoff zeta prefactor per near slot
   size_t cap_bond = 0;
   float4* near = nullptr;   // SW near lists (displacement, entry) per slot
   float4* gsum = nullptr;   // SW per-leg three-body force and energy share
@@ -238,7 +238,7 @@ struct Ctx {
 };
 
 enum : int { TG_BUILD = 0, TG_PASS1 = 1, TG_FORCE = 2, TG_DRIFT = 3, TG_KICK = 4,
-             TG_REDUCE = 5, TG_GRAPH = 6, TG_COUNT = 7 };
+             TG_REDUCE = 5, TG_GRAPH = 6, TG_MLP = 7, TG_COUNT = 8 };
 
 // Brackets the launches of one group with HIP events when timing is on.
 struct TimerScope {

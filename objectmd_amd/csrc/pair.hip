@@ -47,57 +47,72 @@ __device__ inline double shifted(double x, int s, double L) {
   return s == 0 ? x : (s < 0 ? x + (-L) : x + L);
 }
 
+// lanes consecutive threads share one atom (a power of two up to 64): lane j
+// takes list entries j, j + lanes, ...; the partial sums meet by shuffles.
 template <bool ENERGY>
-__global__ void __launch_bounds__(kBlock) k_pair64(PairArgs a) {
-  const int s = xcd_block() * blockDim.x + threadIdx.x;
-  if (s >= a.n) return;
+__global__ void __launch_bounds__(kBlock) k_pair64(PairArgs a, int lanes) {
+  const int g = xcd_block() * blockDim.x + threadIdx.x;
+  const int s = g / lanes, sub = g % lanes;  // lanes | 64: a group never spans waves
+  const bool live = s < a.n;
   const int n = a.n;
-  const double xi = a.x[s], yi = a.x[(size_t)n + s], zi = a.x[2 * (size_t)n + s];
-  const int c = a.cellid[s];
-  const int k[3] = {c % a.M[0], (c / a.M[0]) % a.M[1], c / (a.M[0] * a.M[1])};
-  const int ti = a.type[s];
-  const int nn = a.nnbr[s];
   double fx = 0, fy = 0, fz = 0, pot = 0, vir = 0;
-  for (int e = 0; e < nn; e++) {
-    const uint32_t w = a.nbr[nidx(s, e, n)];
-    const int t = (int)(w & kSlotMask);
-    const int img = (int)(w >> kImgShift);
-    const int o[3] = {img % 3 - 1, (img / 3) % 3 - 1, img / 9 - 1};
-    int sh[3];
-    for (int d = 0; d < 3; d++) {
-      const int kk = k[d] + o[d];
-      sh[d] = ((a.pbc >> d) & 1) ? (kk < 0 ? -1 : (kk >= a.M[d] ? 1 : 0)) : 0;
-    }
-    const int slot = ti * kMaxTypes + a.type[t];
-    const int tb = a.tab[slot];
-    if (tb < 0) continue;
-    double dx, dy, dz, r2;
-    {
-#pragma clang fp contract(off)
-      dx = xi - shifted(a.x[t], sh[0], a.L[0]);
-      dy = yi - shifted(a.x[(size_t)n + t], sh[1], a.L[1]);
-      dz = zi - shifted(a.x[2 * (size_t)n + t], sh[2], a.L[2]);
-      r2 = dx * dx + dy * dy + dz * dz;
-    }
-    if (r2 < 1e-6) {
-      atomicOr(a.err, ERR_OVERLAP);
-      continue;
-    }
-    const double r = sqrt(r2);
-    if (r < a.cut[slot]) {
-      double p, dp;
-      spline_read64(a.tabs[tb], r, p, dp, a.err);
-      const double ff = -dp / r;
-      const double gx = dx * ff, gy = dy * ff, gz = dz * ff;
-      fx += gx;
-      fy += gy;
-      fz += gz;
-      if (ENERGY) {
-        pot += 0.5 * p;
-        vir += gx * dx + gy * dy + gz * dz;
+  if (live) {
+    const double xi = a.x[s], yi = a.x[(size_t)n + s], zi = a.x[2 * (size_t)n + s];
+    const int c = a.cellid[s];
+    const int k[3] = {c % a.M[0], (c / a.M[0]) % a.M[1], c / (a.M[0] * a.M[1])};
+    const int ti = a.type[s];
+    const int nn = a.nnbr[s];
+    for (int e = sub; e < nn; e += lanes) {
+      const uint32_t w = a.nbr[nidx(s, e, n)];
+      const int t = (int)(w & kSlotMask);
+      const int img = (int)(w >> kImgShift);
+      const int o[3] = {img % 3 - 1, (img / 3) % 3 - 1, img / 9 - 1};
+      int sh[3];
+      for (int d = 0; d < 3; d++) {
+        const int kk = k[d] + o[d];
+        sh[d] = ((a.pbc >> d) & 1) ? (kk < 0 ? -1 : (kk >= a.M[d] ? 1 : 0)) : 0;
+      }
+      const int slot = ti * kMaxTypes + a.type[t];
+      const int tb = a.tab[slot];
+      if (tb < 0) continue;
+      double dx, dy, dz, r2;
+      {
+  #pragma clang fp contract(off)
+        dx = xi - shifted(a.x[t], sh[0], a.L[0]);
+        dy = yi - shifted(a.x[(size_t)n + t], sh[1], a.L[1]);
+        dz = zi - shifted(a.x[2 * (size_t)n + t], sh[2], a.L[2]);
+        r2 = dx * dx + dy * dy + dz * dz;
+      }
+      if (r2 < 1e-6) {
+        atomicOr(a.err, ERR_OVERLAP);
+        continue;
+      }
+      const double r = sqrt(r2);
+      if (r < a.cut[slot]) {
+        double p, dp;
+        spline_read64(a.tabs[tb], r, p, dp, a.err);
+        const double ff = -dp / r;
+        const double gx = dx * ff, gy = dy * ff, gz = dz * ff;
+        fx += gx;
+        fy += gy;
+        fz += gz;
+        if (ENERGY) {
+          pot += 0.5 * p;
+          vir += gx * dx + gy * dy + gz * dz;
+        }
       }
     }
   }
+  for (int o = lanes / 2; o > 0; o >>= 1) {
+    fx += __shfl_xor(fx, o, 64);
+    fy += __shfl_xor(fy, o, 64);
+    fz += __shfl_xor(fz, o, 64);
+    if (ENERGY) {
+      pot += __shfl_xor(pot, o, 64);
+      vir += __shfl_xor(vir, o, 64);
+    }
+  }
+  if (!live || sub != 0) return;
   a.f64[s] = fx;
   a.f64[(size_t)n + s] = fy;
   a.f64[2 * (size_t)n + s] = fz;
@@ -207,8 +222,12 @@ void pair_forces(Ctx& c, bool energy) {
     }
   const int nb = grid_blocks(c.n, kBlock);
   if (c.fp64) {
-    if (energy) k_pair64<true><<<nb, kBlock, 0, c.stream>>>(a);
-    else k_pair64<false><<<nb, kBlock, 0, c.stream>>>(a);
+    // small systems: enough lanes per atom for ~128k threads
+    int lanes = 1;
+    while (lanes < 16 && (long)c.n * lanes < 131072) lanes *= 2;
+    const int nbl = grid_blocks((long)c.n * lanes, kBlock);
+    if (energy) k_pair64<true><<<nbl, kBlock, 0, c.stream>>>(a, lanes);
+    else k_pair64<false><<<nbl, kBlock, 0, c.stream>>>(a, lanes);
   } else {
     if (energy) k_pair32<true><<<nb, kBlock, 0, c.stream>>>(a);
     else k_pair32<false><<<nb, kBlock, 0, c.stream>>>(a);

@@ -552,9 +552,12 @@ void surrogate_forces(Ctx& c, bool energy) {
     k_sg_decide<<<1, 1, 0, c.stream>>>(a);
   }
   {
-    TimerScope ts(c, TG_FORCE);
+    TimerScope ts(c, TG_MLP);
     const int groups = (n + 15) / 16;
     k_sg_mlp<<<std::min(2048, std::max(1, (groups + 3) / 4)), kBlock, 0, c.stream>>>(a);
+  }
+  {
+    TimerScope ts(c, TG_FORCE);
     k_sg_centre<<<nb, kBlock, 0, c.stream>>>(a);
     k_sg_legs<<<nb, kBlock, 0, c.stream>>>(a);
   }

@@ -17,11 +17,14 @@
 //             for it; every triplet gives a third of its energy to each of
 //             its atoms, as the reference does
 // Tersoff (PRB 37 6991 with the cutoff of include/potential/tersoff.h:247-257)
-//   pass 1    per bond (i,j): zeta_ij, b_ij, the pair energy 1/2 V_ij and
-//             P_ij = 1/2 fc fA db/dzeta stored per list slot
-//   pass 2    atom l gathers the pair terms of bonds (l,j) and (j,l), the
-//             zeta derivatives of its own bonds, of the bonds (j,l) and of the
-//             bonds (j,k) in whose zeta it appears.
+//   pass 1    list scan to the near list (neighbours inside R + D); per bond
+//             (i,j): zeta_ij, b_ij, the pair energy 1/2 V_ij and
+//             P_ij = 1/2 fc fA db/dzeta stored per near slot
+//   pass 2    per centre i: the gradient of its bond energies with respect
+//             to each leg (pair part, zeta of the leg's own bond, zeta of the
+//             other bonds in which the leg is the k-atom); the centre takes
+//             the sum, each leg's force is stored per near slot
+//   pass 3    the SW gather: each atom adds the leg forces stored for it.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -43,12 +46,13 @@ struct TBArgs {
   float* f;
   float* pot;
   float* vir;
-  float* bond_b;  // Tersoff: b_ij per list slot
-  float* bond_p;  // Tersoff: P_ij per list slot
-  float4* near;   // SW: near list, kMaxNear slot rows of n (x_j - x_l, entry bits)
-  float4* gsum;   // SW: per near entry, force on that leg from the centre's triplets
-  int* near_n;    // SW: near-list length per atom
+  float* bond_b;  // Tersoff: b_ij per near slot
+  float* bond_p;  // Tersoff: P_ij per near slot
+  float4* near;   // near list, kMaxNear slot rows of n (x_j - x_l, entry bits)
+  float4* gsum;   // per near entry, force on that leg from the terms of its centre
+  int* near_n;    // near-list length per atom
   const int* only_if;  // Tersoff passes run only when *only_if != 0 (if set)
+  int leg_min;  // centres with fewer near neighbours store no leg forces (SW 2, Tersoff 1)
   int* err;
   int n;
   float hx, hy, hz;
@@ -66,12 +70,6 @@ struct TBArgs {
   signed char tab[kMaxTypes * kMaxTypes];
   float tcut2[kMaxTypes * kMaxTypes];
   TabView tabs[kMaxTypes * kMaxTypes];
-};
-
-// a neighbour inside the cut (Tersoff passes)
-struct Near {
-  float dx, dy, dz, r;
-  uint32_t e;
 };
 
 __device__ inline float3 disp(const float4& a, const float4& b, const float4& o) {
@@ -329,10 +327,12 @@ __global__ void __launch_bounds__(kBlock) k_sw_centre(TBArgs a) {
   if (ENERGY) a.pot[i] += esum * (1.0f / 6.0f);
 }
 
-// Pass 3: atom l collects, from each near neighbour c, the force and energy
-// share of the triplets centred at c with l as a leg (gsum[c][slot of l]).
+// Pass 3 (SW and Tersoff): atom l collects, from each near neighbour c, the
+// force and energy share of the terms centred at c with l as a leg
+// (gsum[c][slot of l]).
 template <bool ENERGY>
-__global__ void __launch_bounds__(kBlock) k_sw_legs(TBArgs a) {
+__global__ void __launch_bounds__(kBlock) k_tb_legs(TBArgs a) {
+  if (a.only_if && !*a.only_if) return;
   const int l = xcd_block() * blockDim.x + threadIdx.x;
   if (l >= a.n) return;
   const int n = a.n;
@@ -348,7 +348,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_legs(TBArgs a) {
     const float4 du = nl[u];
     const int c = (int)(__float_as_uint(du.w) & kSlotMask);
     const int nc = a.near_n[c];
-    if (nc < 2) continue;
+    if (nc < a.leg_min) continue;
     const Col ncl{a.near + c, (size_t)n};
     for (int s = 0; s < nc; s++) {
       const float4 dk = ncl[s];  // x_k - x_c
@@ -412,6 +412,22 @@ __device__ inline float ters_zeta_term(const TersoffParams& p, float l3m, const 
   return fck * g * ex;
 }
 
+// Pure pair part of a Tersoff bond: 1/2 dV/dr / r at fixed b and, for the
+// energy, 1/2 V = 1/2 fc (fR + b fA).
+__device__ inline float ters_pair(const TersoffParams& p, float r, float b, float* ehalf) {
+  float fc, dfc;
+  ters_fc(p, r, fc, dfc);
+  const float fR = p.A * expf(-p.lambda1 * r);
+  const float fA = -p.B * expf(-p.lambda2 * r);
+  if (ehalf) *ehalf = 0.5f * fc * (fR + b * fA);
+  const float dV = dfc * (fR + b * fA) + fc * (-p.lambda1 * fR - p.lambda2 * b * fA);
+  return 0.5f * dV / r;
+}
+
+// Pass 1: the list scan writes the near list {x_j - x_i, entry} of the
+// neighbours inside R + D (slot-major, as the SW passes); then per near bond
+// (i,j): zeta_ij, b_ij and P_ij = 1/2 fc fA db/dzeta, stored per near slot,
+// and the bond energies 1/2 V_ij given to the centre.
 template <bool ENERGY>
 __global__ void __launch_bounds__(kBlock) k_tersoff_bond(TBArgs a) {
   if (a.only_if && !*a.only_if) return;  // uniform: a step of the surrogate
@@ -425,37 +441,36 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_bond(TBArgs a) {
   const float l3m = p.lambda3 * p.lambda3 * p.lambda3;
   const float4 qi = a.q[i];
   const int cnt = a.nnbr[i];
-  Near nr[kMaxNear];
-  int slot[kMaxNear];
+  const Col ni{a.near + i, (size_t)n};
   int nn = 0;
   for (int k = 0; k < cnt; k++) {
     const uint32_t e = a.nbr[nidx(i, k, n)];
     const float3 d = disp(qi, a.q[e & kSlotMask], offs[e >> kImgShift]);
     const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
-    if (r2 < 1e-6f) atomicOr(a.err, ERR_OVERLAP);
-    if (r2 < a.rc2 && r2 >= 1e-6f) {
-      if (nn < kMaxNear) {
-        nr[nn] = {d.x, d.y, d.z, sqrtf(r2), e};
-        slot[nn] = k;
-      } else {
-        atomicOr(a.err, ERR_NEAR);
-      }
+    if (r2 < 1e-6f) {
+      atomicOr(a.err, ERR_OVERLAP);
+      continue;
+    }
+    if (r2 < a.rc2) {
+      if (nn < kMaxNear) ni[nn] = make_float4(d.x, d.y, d.z, __uint_as_float(e));
+      else atomicOr(a.err, ERR_NEAR);
       nn++;
-    } else {
-      a.bond_b[(size_t)k * n + i] = 0.f;
-      a.bond_p[(size_t)k * n + i] = 0.f;
     }
   }
   if (nn > kMaxNear) nn = kMaxNear;
+  a.near_n[i] = nn;
   float pot = 0.f;
   for (int u = 0; u < nn; u++) {
-    const float dij[3] = {nr[u].dx, nr[u].dy, nr[u].dz};
-    const float rij = nr[u].r;
+    const float4 du = ni[u];
+    const float dij[3] = {du.x, du.y, du.z};
+    const float rij = sqrtf(du.x * du.x + du.y * du.y + du.z * du.z);
     float zeta = 0.f;
     for (int w = 0; w < nn; w++) {
       if (w == u) continue;
-      const float dik[3] = {nr[w].dx, nr[w].dy, nr[w].dz};
-      zeta += ters_zeta_term(p, l3m, dij, rij, dik, nr[w].r, nullptr, nullptr);
+      const float4 dw = ni[w];
+      const float dik[3] = {dw.x, dw.y, dw.z};
+      const float rik = sqrtf(dw.x * dw.x + dw.y * dw.y + dw.z * dw.z);
+      zeta += ters_zeta_term(p, l3m, dij, rij, dik, rik, nullptr, nullptr);
     }
     float fc, dfc;
     ters_fc(p, rij, fc, dfc);
@@ -479,8 +494,8 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_bond(TBArgs a) {
       b = expf(-0.5f / p.n * l1p);
       db = -0.5f * b * t / zeta;
     }
-    a.bond_b[(size_t)slot[u] * n + i] = b;
-    a.bond_p[(size_t)slot[u] * n + i] = 0.5f * fc * fA * db;
+    a.bond_b[(size_t)u * n + i] = b;
+    a.bond_p[(size_t)u * n + i] = 0.5f * fc * fA * db;
     if (ENERGY) {
       const float fR = p.A * expf(-p.lambda1 * rij);
       pot += 0.5f * fc * (fR + b * fA);
@@ -492,126 +507,57 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_bond(TBArgs a) {
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_tersoff_force(TBArgs a) {
+// Pass 2: the bonds centred at each atom i.  With d_u = x_j(u) - x_i the
+// gradient of E_i = sum_u 1/2 V(r_u, b_u) with respect to d_u is
+//   S_u = pair_u d_u + P_u sum_{w != u} dT(d_u, d_w)/dd_u
+//                    + sum_{w != u} P_w dT(d_w, d_u)/dd_u,
+// leg u receives -S_u (stored per near slot for pass 3) and the centre
+// +sum_u S_u.
+__global__ void __launch_bounds__(kBlock) k_tersoff_centre(TBArgs a) {
   if (a.only_if && !*a.only_if) return;
-  __shared__ float4 offs[27];
-  load_offsets(offs, a.hx, a.hy, a.hz);
-  __syncthreads();
-  const int l = xcd_block() * blockDim.x + threadIdx.x;
-  if (l >= a.n) return;
+  const int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
   const int n = a.n;
   const TersoffParams& p = a.ts;
   const float l3m = p.lambda3 * p.lambda3 * p.lambda3;
-  const float4 ql = a.q[l];
-  const int cnt = a.nnbr[l];
-  Near nr[kMaxNear];
-  int slot[kMaxNear];
-  int nn = 0;
-  for (int k = 0; k < cnt && nn < kMaxNear; k++) {
-    const uint32_t e = a.nbr[nidx(l, k, n)];
-    const float3 d = disp(ql, a.q[e & kSlotMask], offs[e >> kImgShift]);
-    const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
-    if (r2 < a.rc2 && r2 >= 1e-6f) {
-      nr[nn] = {d.x, d.y, d.z, sqrtf(r2), e};
-      slot[nn] = k;
-      nn++;
-    }
-  }
+  const int nn = a.near_n[i];
+  const Col ni{a.near + i, (size_t)n}, gi{a.gsum + i, (size_t)n};
   Kahan fx, fy, fz;
-  auto addf = [&](float gx, float gy, float gz) {
-    fx.add(gx);
-    fy.add(gy);
-    fz.add(gz);
-  };
-  // (1) bonds centred at l: pair part and zeta derivative w.r.t. the centre
   for (int u = 0; u < nn; u++) {
-    const float dij[3] = {nr[u].dx, nr[u].dy, nr[u].dz};
-    const float rij = nr[u].r;
-    const float b = a.bond_b[(size_t)slot[u] * n + l];
-    const float P = a.bond_p[(size_t)slot[u] * n + l];
-    float fc, dfc;
-    ters_fc(p, rij, fc, dfc);
-    const float fR = p.A * expf(-p.lambda1 * rij);
-    const float fA = -p.B * expf(-p.lambda2 * rij);
-    const float dV = dfc * (fR + b * fA) + fc * (-p.lambda1 * fR - p.lambda2 * b * fA);
-    const float pref = 0.5f * dV / rij;  // force on l: +pref * dij
-    addf(pref * dij[0], pref * dij[1], pref * dij[2]);
-    if (P != 0.f) {
-      float sx = 0.f, sy = 0.f, sz = 0.f;
-      for (int w = 0; w < nn; w++) {
-        if (w == u) continue;
-        const float dik[3] = {nr[w].dx, nr[w].dy, nr[w].dz};
-        float gj[3], gk[3];
-        ters_zeta_term(p, l3m, dij, rij, dik, nr[w].r, gj, gk);
-        sx += gj[0] + gk[0];
-        sy += gj[1] + gk[1];
-        sz += gj[2] + gk[2];
-      }
-      addf(P * sx, P * sy, P * sz);
-    }
-  }
-  // (2) bonds centred at a neighbour j: l as the j-atom (pair part of bond
-  //     (j,l) and its zeta derivative) and l as a k-atom of bonds (j,k)
-  for (int u = 0; u < nn; u++) {
-    const uint32_t ej = nr[u].e;
-    const int j = (int)(ej & kSlotMask);
-    const float4 qj = a.q[j];
-    const float djl[3] = {-nr[u].dx, -nr[u].dy, -nr[u].dz};  // x_l - x_j
-    const float rjl = nr[u].r;
-    const int cj = a.nnbr[j];
-    // neighbours of j inside the cut (the bond (j,l) is among them)
-    Near mj[kMaxNear];
-    int ms[kMaxNear];
-    int mm = 0, self = -1;
-    for (int k = 0; k < cj && mm < kMaxNear; k++) {
-      const uint32_t e = a.nbr[nidx(j, k, n)];
-      const int t = (int)(e & kSlotMask);
-      const float3 d = disp(qj, a.q[t], offs[e >> kImgShift]);  // x_t - x_j
-      const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
-      if (!(r2 < a.rc2) || r2 < 1e-6f) continue;
-      if (t == l && self < 0) {
-        const float ex = d.x - djl[0], ey = d.y - djl[1], ez = d.z - djl[2];
-        if (ex * ex + ey * ey + ez * ez < 1e-4f) self = mm;
-      }
-      mj[mm] = {d.x, d.y, d.z, sqrtf(r2), e};
-      ms[mm] = k;
-      mm++;
-    }
-    if (self < 0) continue;  // cannot happen for a symmetric list
-    // pair part of bond (j,l): force on l = -pref_jl * d_jl(x_l - x_j)
-    {
-      const float b = a.bond_b[(size_t)ms[self] * n + j];
-      float fc, dfc;
-      ters_fc(p, rjl, fc, dfc);
-      const float fR = p.A * expf(-p.lambda1 * rjl);
-      const float fA = -p.B * expf(-p.lambda2 * rjl);
-      const float dV = dfc * (fR + b * fA) + fc * (-p.lambda1 * fR - p.lambda2 * b * fA);
-      const float pref = 0.5f * dV / rjl;
-      addf(-pref * djl[0], -pref * djl[1], -pref * djl[2]);
-    }
-    const float Pjl = a.bond_p[(size_t)ms[self] * n + j];
-    float gsum[3] = {0.f, 0.f, 0.f};
-    for (int w = 0; w < mm; w++) {
-      if (w == self) continue;
-      const float djk[3] = {mj[w].dx, mj[w].dy, mj[w].dz};
+    const float4 du = ni[u];
+    const float dij[3] = {du.x, du.y, du.z};
+    const float rij = sqrtf(du.x * du.x + du.y * du.y + du.z * du.z);
+    const float Pu = a.bond_p[(size_t)u * n + i];
+    const float pr = ters_pair(p, rij, a.bond_b[(size_t)u * n + i], nullptr);
+    float sx = pr * dij[0], sy = pr * dij[1], sz = pr * dij[2];
+    for (int w = 0; w < nn; w++) {
+      if (w == u) continue;
+      const float4 dw = ni[w];
+      const float dik[3] = {dw.x, dw.y, dw.z};
+      const float rik = sqrtf(dw.x * dw.x + dw.y * dw.y + dw.z * dw.z);
       float gj[3], gk[3];
-      // bond (j,l) with k = w: gradient w.r.t. l as the j-atom
-      ters_zeta_term(p, l3m, djl, rjl, djk, mj[w].r, gj, gk);
-      gsum[0] += gj[0];
-      gsum[1] += gj[1];
-      gsum[2] += gj[2];
-      // bond (j,w) with l as the k-atom
-      const float Pjk = a.bond_p[(size_t)ms[w] * n + j];
-      if (Pjk != 0.f) {
-        ters_zeta_term(p, l3m, djk, mj[w].r, djl, rjl, gj, gk);
-        addf(-Pjk * gk[0], -Pjk * gk[1], -Pjk * gk[2]);
+      if (Pu != 0.f) {
+        ters_zeta_term(p, l3m, dij, rij, dik, rik, gj, gk);  // bond u, u as the j-atom
+        sx += Pu * gj[0];
+        sy += Pu * gj[1];
+        sz += Pu * gj[2];
+      }
+      const float Pw = a.bond_p[(size_t)w * n + i];
+      if (Pw != 0.f) {
+        ters_zeta_term(p, l3m, dik, rik, dij, rij, gj, gk);  // bond w, u as a k-atom
+        sx += Pw * gk[0];
+        sy += Pw * gk[1];
+        sz += Pw * gk[2];
       }
     }
-    addf(-Pjl * gsum[0], -Pjl * gsum[1], -Pjl * gsum[2]);
+    gi[u] = make_float4(-sx, -sy, -sz, 0.f);
+    fx.add(sx);
+    fy.add(sy);
+    fz.add(sz);
   }
-  a.f[l] = fx.s;
-  a.f[(size_t)n + l] = fy.s;
-  a.f[2 * (size_t)n + l] = fz.s;
+  a.f[i] = fx.s;
+  a.f[(size_t)n + i] = fy.s;
+  a.f[2 * (size_t)n + i] = fz.s;
 }
 
 TBArgs base_args(Ctx& c) {
@@ -709,6 +655,7 @@ void sw_forces(Ctx& c, bool energy) {
   a.near = c.near;
   a.gsum = c.gsum;
   a.near_n = c.near_n;
+  a.leg_min = 2;  // the centre pass stores nothing for centres without a triplet
   const int nb = grid_blocks(c.n, kBlock);
   {
     TimerScope ts(c, TG_PASS1);
@@ -719,10 +666,10 @@ void sw_forces(Ctx& c, bool energy) {
     TimerScope ts(c, TG_FORCE);
     if (energy) {
       k_sw_centre<true><<<nb, kBlock, 0, c.stream>>>(a);
-      k_sw_legs<true><<<nb, kBlock, 0, c.stream>>>(a);
+      k_tb_legs<true><<<nb, kBlock, 0, c.stream>>>(a);
     } else {
       k_sw_centre<false><<<nb, kBlock, 0, c.stream>>>(a);
-      k_sw_legs<false><<<nb, kBlock, 0, c.stream>>>(a);
+      k_tb_legs<false><<<nb, kBlock, 0, c.stream>>>(a);
     }
   }
   OMDG_HIP(hipGetLastError());
@@ -734,7 +681,7 @@ void tersoff_forces_if(Ctx& c, bool energy, const int* only_if) {
   a.ts = c.ters;
   a.rc = c.ters.R + c.ters.D;
   a.rc2 = a.rc * a.rc;
-  const size_t need = (size_t)c.maxnbr * (size_t)c.n;
+  const size_t need = (size_t)kMaxNear * (size_t)c.n;
   if (need > c.cap_bond) {
     if (c.bond_b) (void)hipFree(c.bond_b);
     if (c.bond_p) (void)hipFree(c.bond_p);
@@ -745,6 +692,11 @@ void tersoff_forces_if(Ctx& c, bool energy, const int* only_if) {
   }
   a.bond_b = c.bond_b;
   a.bond_p = c.bond_p;
+  near_arrays(c, kMaxNear);
+  a.near = c.near;
+  a.gsum = c.gsum;
+  a.near_n = c.near_n;
+  a.leg_min = 1;  // a lone bond still has its pair term
   const int nb = grid_blocks(c.n, kBlock);
   {
     TimerScope ts(c, TG_PASS1);
@@ -753,7 +705,8 @@ void tersoff_forces_if(Ctx& c, bool energy, const int* only_if) {
   }
   {
     TimerScope ts(c, TG_FORCE);
-    k_tersoff_force<<<nb, kBlock, 0, c.stream>>>(a);
+    k_tersoff_centre<<<nb, kBlock, 0, c.stream>>>(a);
+    k_tb_legs<false><<<nb, kBlock, 0, c.stream>>>(a);
   }
   OMDG_HIP(hipGetLastError());
 }

@@ -58,8 +58,14 @@ def load(path=MODEL_PATH):
 
 
 def api_blocks(model, desc):
-    """(desc[22], norm[128], weights[6337]) in the order of omdg_set_surrogate."""
-    norm = np.concatenate([model["mean"], model["std"], model["lo"], model["hi"]])
+    """(desc[22], norm[128], weights[6337]) in the order of omdg_set_surrogate.
+    The gauge box of a descriptor the training data left nearly constant is
+    widened to 5 % of the widest one, so that it measures distance on the
+    common scale rather than flagging every small excursion."""
+    lo, hi = np.asarray(model["lo"], float), np.asarray(model["hi"], float)
+    width = np.maximum(hi - lo, 0.05 * float((hi - lo).max()))
+    mid = 0.5 * (lo + hi)
+    norm = np.concatenate([model["mean"], model["std"], mid - 0.5 * width, mid + 0.5 * width])
     w = np.concatenate([np.ravel(model["W1"]), model["b1"], np.ravel(model["W2"]), model["b2"],
                         model["w3"], np.atleast_1d(model["b3"])]).astype(np.float32)
     return desc_vector(desc), norm.astype(np.float64), w

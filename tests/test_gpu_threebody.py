@@ -126,6 +126,30 @@ def test_tersoff_forces_vs_oracle(pbc, R, D):
     eng.close()
 
 
+def test_tersoff_lone_bonds():
+    """A bent trimer and a dimer: atoms whose only bond carries the whole
+    pair term (regression for leg forces of single-bond centres)."""
+    x = np.array([[5.0, 5.0, 5.0], [7.3, 5.0, 5.0], [8.5, 7.0, 5.2],
+                  [15.0, 15.0, 15.0], [17.4, 15.1, 15.0]])
+    lo, hi = np.zeros(3), np.full(3, 22.0)
+    eng = om.Engine()
+    eng.set_box(lo, hi, 0)
+    eng.upload_atoms(x, mass=[tables.mass_sim(tables.MASS_SI)])
+    eng.set_tersoff(tersoff_params(2.7, 0.15))
+    eng.set_potential(om.POT_TERSOFF)
+    eng.set_neighbor(0.3, 10, 0)
+    ep, _ = eng.compute()
+    f = eng.download(om.F)
+    nb = O.neighbors(x, lo, hi - lo, 0, 2.85 + 0.3, maxn=8)
+    fo, po = O.tersoff_forces(x, lo, hi - lo, 0, nb, _ters_oracle(2.7, 0.15))
+    err = rel_rms(f, fo)
+    say(f"[tersoff-lone] relRMS={err:.3e} Ep={ep:.8f} oracle={po.sum():.8f}")
+    assert err <= FORCE_RTOL
+    assert np.abs(f.sum(axis=0)).max() < 1e-4
+    assert ep == pytest.approx(po.sum(), rel=1e-6)
+    eng.close()
+
+
 def test_tersoff_steps_conserve_energy():
     x, lo, hi = _si(5, 0.05, 3, 3, a=5.432)
     m = tables.mass_sim(tables.MASS_SI)
