@@ -138,9 +138,9 @@ def bench_eam(args):
     log(f"[bench] {steps} steps in {elapsed:.4f} s: {rate:.4e} atom-steps/s")
     if args.copy_probe_mb:
         copy_probe(args.copy_probe_mb)
-    # force pass: read the FP32 snapshot (16 B) and F'(rho) (4 B), write the
-    # FP32 force (12 B) per atom
-    achieved, _ = kernel_rate(times, "force", 32 * n)
+    # force pass: read the FP32 snapshot with F'(rho) (16 B), write the FP32
+    # force (12 B) per atom
+    achieved, _ = kernel_rate(times, "force", 28 * n)
     force_err = None
     if not args.no_force_check:
         from oracle import oracle as O
@@ -300,7 +300,8 @@ def bench_tersoff(args):
                 "atoms": n, "force_dtype": "f32 with compensated per-atom sums",
                 "integrator_dtype": "f64", "force_rel_rms_vs_fp64": force_err,
                 "skin_A": skin},
-        roofline={"kernel": "k_tersoff_force", "achieved": achieved, "traffic": None},
+        roofline={"kernel": "k_tersoff_centre + k_tb_legs", "achieved": achieved,
+                  "traffic": None},
         times=times, cpu=cpu)
 
 
@@ -373,7 +374,8 @@ def bench_impact(args):
                 "atoms": n, "replicas": nrep, "force_dtype": "f32 with compensated sums",
                 "integrator_dtype": "f64", "force_rel_rms_vs_fp64": force_err,
                 "skin_A": skin, "simulated_fs_min_max": [1e3 * t.min(), 1e3 * t.max()]},
-        roofline={"kernel": "k_sw_three", "achieved": achieved, "traffic": None},
+        roofline={"kernel": "k_sw_centre + k_tb_legs", "achieved": achieved,
+                  "traffic": None},
         times=times, cpu=cpu)
 
 

@@ -229,13 +229,21 @@ __device__ inline double shifted(double x, int s, double L) {
   return s == 0 ? x : (s < 0 ? x + (-L) : x + L);
 }
 
+// L lanes per atom test L consecutive candidates of a cell per round and
+// append the members in candidate order (positions from a wavefront
+// ballot): every L writes the same list.
+template <int L>
 __global__ void __launch_bounds__(kBlock) k_batch_build(BBArgs a) {
 #pragma clang fp contract(off)
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = gid / L, j = gid % L;
+  // whole lane groups leave together (L divides 64)
   if (s >= a.n || !a.need[a.rep[s]]) return;
+  const int gbase = (int)(threadIdx.x & 63) - j;
+  const unsigned below = (1u << j) - 1u;
   const int n = a.n;
   if (a.has_flags && !(a.flag[s] & OMDG_FLAG_ACTIVE)) {
-    a.nnbr[s] = 0;
+    if (j == 0) a.nnbr[s] = 0;
     return;
   }
   const RepGrid& g = a.grid[a.rep[s]];
@@ -275,36 +283,61 @@ __global__ void __launch_bounds__(kBlock) k_batch_build(BBArgs a) {
         const bool noshift = (sx | sy | sz) == 0;
         const float ox = (float)dx * hx, oy = (float)dy * hy, oz = (float)dz * hz;
         const int e = a.start[cc + 1];
-        for (int u = a.start[cc]; u < e; u++) {
-          const float4 qt = a.qs[u];
-          const uint32_t w = __float_as_uint(qt.w);
-          const int t = (int)(w & kSlotMask);
-          if ((t == s && noshift) || !(w & kCandActive)) continue;
-          const int pt = ti * kMaxTypes + (int)((w >> kCandTypeShift) & 3u);
-          const float fx = qi.x - qt.x - ox, fy = qi.y - qt.y - oy, fz = qi.z - qt.z - oz;
-          const float r2f = fx * fx + fy * fy + fz * fz;
-          bool in;
-          if (r2f < a.rl2_lo[pt]) {
-            in = true;
-          } else if (r2f > a.rl2_hi[pt]) {
-            in = false;
-          } else {
-            const double ddx = xi - shifted(a.x[t], sx, g.L[0]);
-            const double ddy = yi - shifted(a.x[(size_t)n + t], sy, g.L[1]);
-            const double ddz = zi - shifted(a.x[2 * (size_t)n + t], sz, g.L[2]);
-            in = ddx * ddx + ddy * ddy + ddz * ddz <= a.rl2[pt];
+        for (int u0 = a.start[cc]; u0 < e; u0 += L) {
+          const int u = u0 + j;
+          bool in = false;
+          int t = 0;
+          if (u < e) {
+            const float4 qt = a.qs[u];
+            const uint32_t w = __float_as_uint(qt.w);
+            t = (int)(w & kSlotMask);
+            if (!(t == s && noshift) && (w & kCandActive)) {
+              const int pt = ti * kMaxTypes + (int)((w >> kCandTypeShift) & 3u);
+              const float fx = qi.x - qt.x - ox, fy = qi.y - qt.y - oy, fz = qi.z - qt.z - oz;
+              const float r2f = fx * fx + fy * fy + fz * fz;
+              if (r2f < a.rl2_lo[pt]) {
+                in = true;
+              } else if (r2f <= a.rl2_hi[pt]) {
+                const double ddx = xi - shifted(a.x[t], sx, g.L[0]);
+                const double ddy = yi - shifted(a.x[(size_t)n + t], sy, g.L[1]);
+                const double ddz = zi - shifted(a.x[2 * (size_t)n + t], sz, g.L[2]);
+                in = ddx * ddx + ddy * ddy + ddz * ddz <= a.rl2[pt];
+              }
+            }
           }
-          if (in) {
-            if (cnt < a.maxnbr) a.nbr[nidx(s, cnt, n)] = (uint32_t)t | (img << kImgShift);
-            cnt++;
+          if (L == 1) {
+            if (in) {
+              if (cnt < a.maxnbr) a.nbr[nidx(s, cnt, n)] = (uint32_t)t | (img << kImgShift);
+              cnt++;
+            }
+          } else {
+            const unsigned gm =
+                (unsigned)(__ballot(in) >> gbase) & (unsigned)((1ull << L) - 1ull);
+            if (in) {
+              const int pos = cnt + __popc(gm & below);
+              if (pos < a.maxnbr) a.nbr[nidx(s, pos, n)] = (uint32_t)t | (img << kImgShift);
+            }
+            cnt += __popc(gm);
           }
         }
       }
     }
   }
-  a.nnbr[s] = cnt < a.maxnbr ? cnt : a.maxnbr;
-  if (cnt > a.maxnbr) atomicOr(a.stat + ST_ERR, ERR_NBR_OVERFLOW);
-  atomicMax(a.stat + ST_MAXNBR, cnt);
+  if (j == 0) {
+    a.nnbr[s] = cnt < a.maxnbr ? cnt : a.maxnbr;
+    if (cnt > a.maxnbr) atomicOr(a.stat + ST_ERR, ERR_NBR_OVERFLOW);
+    atomicMax(a.stat + ST_MAXNBR, cnt);
+  }
+}
+
+void launch_batch_build(Ctx& c, const BBArgs& a, int lanes) {
+  const long t = (long)a.n * lanes;
+  switch (lanes) {
+    case 16: k_batch_build<16><<<grid_blocks(t, kBlock), kBlock, 0, c.stream>>>(a); break;
+    case 8: k_batch_build<8><<<grid_blocks(t, kBlock), kBlock, 0, c.stream>>>(a); break;
+    case 4: k_batch_build<4><<<grid_blocks(t, kBlock), kBlock, 0, c.stream>>>(a); break;
+    default: k_batch_build<1><<<grid_blocks(t, kBlock), kBlock, 0, c.stream>>>(a); break;
+  }
 }
 
 // ---- per-replica kernels ----------------------------------------------------
@@ -563,7 +596,10 @@ void batch_build(Ctx& c, bool all) {
   k_batch_snapshot<<<grid_blocks(n, kBlock), kBlock, 0, c.stream>>>(a);
   k_batch_cell_sort<<<grid_blocks((long)nc * 64, 256), 256, 0, c.stream>>>(
       c.perm, c.cell_start, nc, c.cell_rep, c.rep_need, c.q, c.flag[c.cur], a.has_flags, c.qs);
-  k_batch_build<<<grid_blocks(n, kBlock), kBlock, 0, c.stream>>>(a);
+  // the lane count is timed on the first full build (never under capture)
+  if (c.build_lanes < 0 && all)
+    c.build_lanes = pick_build_lanes(c, n, [&](int l) { launch_batch_build(c, a, l); });
+  launch_batch_build(c, a, c.build_lanes < 0 ? 4 : c.build_lanes);
   OMDG_HIP(hipGetLastError());
 }
 

@@ -45,7 +45,7 @@ struct EamArgs {
   const uint32_t* nbr;
   const int* nnbr;
   float* rho;
-  float* fp;
+  float4* qf;  // density pass: {q.xyz, F'(rho)} per atom for the force pass
   float* emb;
   float* f;  // 3 planes
   float* pot;
@@ -67,8 +67,17 @@ __device__ inline double lanes_sum(const Kahan& k) {
   return v;
 }
 
+// list words one lane requests before using any: a stretch of LANES * kPre
+// entries costs one memory latency instead of one per CHUNK
+template <int LANES, int CHUNK>
+constexpr int pre_words() {
+  constexpr int w = 64 / LANES > 16 ? 16 : 64 / LANES;
+  return w < CHUNK ? CHUNK : w;
+}
+
 template <bool ENERGY, int LANES, int CHUNK>
 __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
+  constexpr int kPre = pre_words<LANES, CHUNK>();
   __shared__ float4 offs[27];
   load_offsets(offs, a.hx, a.hy, a.hz);
   __syncthreads();
@@ -80,38 +89,46 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
   const int nn = live ? a.nnbr[s] : 0;
   int err = 0;
   Kahan rho;
-  for (int k0 = j; k0 < nn; k0 += LANES * CHUNK) {
-    uint32_t e[CHUNK];
+  for (int b0 = j; b0 < nn; b0 += LANES * kPre) {
+    // all list words of this stretch in flight at once
+    uint32_t ew[kPre];
 #pragma unroll
-    for (int u = 0; u < CHUNK; u++) {
-      const int k = k0 + u * LANES;
-      e[u] = (k < nn) ? a.nbr[nidx(s, k, n)] : kNone;
+    for (int t = 0; t < kPre; t++) {
+      const int k = b0 + t * LANES;
+      ew[t] = (k < nn) ? a.nbr[nidx(s, k, n)] : kNone;
     }
-    float4 qj[CHUNK];
 #pragma unroll
-    for (int u = 0; u < CHUNK; u++) qj[u] = (e[u] != kNone) ? a.q[e[u] & kSlotMask] : qi;
-    float r[CHUNK], dx[CHUNK];
-    bool in[CHUNK];
-    int seg[CHUNK];
+    for (int t0 = 0; t0 < kPre; t0 += CHUNK) {
+      if (b0 + t0 * LANES >= nn) break;
+      uint32_t e[CHUNK];
 #pragma unroll
-    for (int u = 0; u < CHUNK; u++) {
-      const float4 o = offs[e[u] != kNone ? (e[u] >> kImgShift) : kCentre];
-      const float ex = qi.x - qj[u].x - o.x;
-      const float ey = qi.y - qj[u].y - o.y;
-      const float ez = qi.z - qj[u].z - o.z;
-      const float r2 = ex * ex + ey * ey + ez * ez;
-      in[u] = (e[u] != kNone) && (r2 <= a.cut_e2);
-      r[u] = in[u] ? r2 * rsqrtf(r2) : a.edens.r0;
-      seg[u] = spline_locate32_nb(a.edens, r[u], dx[u], err);
-    }
-    float4 c[CHUNK];
+      for (int u = 0; u < CHUNK; u++) e[u] = ew[t0 + u];
+      float4 qj[CHUNK];
 #pragma unroll
-    for (int u = 0; u < CHUNK; u++) c[u] = a.edens.c32[seg[u]];
+      for (int u = 0; u < CHUNK; u++) qj[u] = (e[u] != kNone) ? a.q[e[u] & kSlotMask] : qi;
+      float r[CHUNK], dx[CHUNK];
+      bool in[CHUNK];
+      int seg[CHUNK];
 #pragma unroll
-    for (int u = 0; u < CHUNK; u++) {
-      float v = c[u].w + dx[u] * (c[u].z + dx[u] * (c[u].y + dx[u] * c[u].x));
-      if (a.edens.rmult) v /= r[u];
-      rho.add(in[u] ? v : 0.f);
+      for (int u = 0; u < CHUNK; u++) {
+        const float4 o = offs[e[u] != kNone ? (e[u] >> kImgShift) : kCentre];
+        const float ex = qi.x - qj[u].x - o.x;
+        const float ey = qi.y - qj[u].y - o.y;
+        const float ez = qi.z - qj[u].z - o.z;
+        const float r2 = ex * ex + ey * ey + ez * ez;
+        in[u] = (e[u] != kNone) && (r2 <= a.cut_e2);
+        r[u] = in[u] ? r2 * rsqrtf(r2) : a.edens.r0;
+        seg[u] = spline_locate32_nb(a.edens, r[u], dx[u], err);
+      }
+      float4 c[CHUNK];
+#pragma unroll
+      for (int u = 0; u < CHUNK; u++) c[u] = a.edens.c32[seg[u]];
+#pragma unroll
+      for (int u = 0; u < CHUNK; u++) {
+        float v = c[u].w + dx[u] * (c[u].z + dx[u] * (c[u].y + dx[u] * c[u].x));
+        if (a.edens.rmult) v /= r[u];
+        rho.add(in[u] ? v : 0.f);
+      }
     }
   }
   if (err) atomicOr(a.err, err);
@@ -121,13 +138,14 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
     // embedding function and derivative in FP64, once per atom
     double F, dF;
     spline_read64(a.embed, (double)(float)rs, F, dF, a.err);
-    a.fp[s] = (float)dF;
+    a.qf[s] = make_float4(qi.x, qi.y, qi.z, (float)dF);
     if (ENERGY) a.emb[s] = (float)F;
   }
 }
 
 template <bool ENERGY, bool PACKED, int LANES, int CHUNK>
 __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
+  constexpr int kPre = pre_words<LANES, CHUNK>();
   __shared__ float4 offs[27];
   load_offsets(offs, a.hx, a.hy, a.hz);
   __syncthreads();
@@ -135,82 +153,85 @@ __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
   const int s = g / LANES, j = g % LANES;
   const int n = a.n;
   const bool live = s < n;
-  const float4 qi = live ? a.q[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-  const float fpi = live ? a.fp[s] : 0.f;
+  const float4 qi = live ? a.qf[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float fpi = qi.w;
   const int nn = live ? a.nnbr[s] : 0;
   int err = 0;
   Kahan fx, fy, fz, pot, vir;
-  for (int k0 = j; k0 < nn; k0 += LANES * CHUNK) {
-    uint32_t e[CHUNK];
+  for (int b0 = j; b0 < nn; b0 += LANES * kPre) {
+    // all list words of this stretch in flight at once
+    uint32_t ew[kPre];
 #pragma unroll
-    for (int u = 0; u < CHUNK; u++) {
-      const int k = k0 + u * LANES;
-      e[u] = (k < nn) ? a.nbr[nidx(s, k, n)] : kNone;
-    }
-    float4 qj[CHUNK];
-    float fpj[CHUNK];
-#pragma unroll
-    for (int u = 0; u < CHUNK; u++) {
-      const bool v = e[u] != kNone;
-      qj[u] = v ? a.q[e[u] & kSlotMask] : qi;
-      fpj[u] = v ? a.fp[e[u] & kSlotMask] : 0.f;
-    }
-    float d[CHUNK][3], r2[CHUNK], rinv[CHUNK], dxp[CHUNK], dxe[CHUNK];
-    bool in[CHUNK], ine[CHUNK];
-    int sp[CHUNK], se[CHUNK];
-#pragma unroll
-    for (int u = 0; u < CHUNK; u++) {
-      const float4 o = offs[e[u] != kNone ? (e[u] >> kImgShift) : kCentre];
-      d[u][0] = qi.x - qj[u].x - o.x;
-      d[u][1] = qi.y - qj[u].y - o.y;
-      d[u][2] = qi.z - qj[u].z - o.z;
-      r2[u] = d[u][0] * d[u][0] + d[u][1] * d[u][1] + d[u][2] * d[u][2];
-      in[u] = (e[u] != kNone) && (r2[u] <= a.cut_p2);
-      ine[u] = in[u] && (r2[u] <= a.cut_e2);
-      rinv[u] = in[u] ? rsqrtf(r2[u]) : 1.f;
-      const float r = in[u] ? r2[u] * rinv[u] : a.phi.r0;
-      sp[u] = spline_locate32_nb(a.phi, r, dxp[u], err);
-      if (PACKED) {
-        se[u] = sp[u];
-        dxe[u] = dxp[u];
-      } else {
-        se[u] = spline_locate32_nb(a.edens, ine[u] ? r : a.edens.r0, dxe[u], err);
-      }
-    }
-    float4 cp[CHUNK], ce[CHUNK];
-#pragma unroll
-    for (int u = 0; u < CHUNK; u++) {
-      if (PACKED) {
-        cp[u] = a.pack[2 * sp[u]];
-        ce[u] = a.pack[2 * sp[u] + 1];
-      } else {
-        cp[u] = a.phi.c32[sp[u]];
-        ce[u] = a.edens.c32[se[u]];
-      }
+    for (int t = 0; t < kPre; t++) {
+      const int k = b0 + t * LANES;
+      ew[t] = (k < nn) ? a.nbr[nidx(s, k, n)] : kNone;
     }
 #pragma unroll
-    for (int u = 0; u < CHUNK; u++) {
-      const float x = dxp[u];
-      float p = cp[u].w + x * (cp[u].z + x * (cp[u].y + x * cp[u].x));
-      float dp = cp[u].z + x * (2.0f * cp[u].y + 3.0f * x * cp[u].x);
-      if (a.phi.rmult) {
-        p *= rinv[u];
-        dp = (dp - p) * rinv[u];
+    for (int t0 = 0; t0 < kPre; t0 += CHUNK) {
+      if (b0 + t0 * LANES >= nn) break;
+      uint32_t e[CHUNK];
+#pragma unroll
+      for (int u = 0; u < CHUNK; u++) e[u] = ew[t0 + u];
+      float4 qj[CHUNK];
+#pragma unroll
+      for (int u = 0; u < CHUNK; u++) qj[u] = (e[u] != kNone) ? a.qf[e[u] & kSlotMask] : qi;
+      float d[CHUNK][3], r2[CHUNK], rinv[CHUNK], dxp[CHUNK], dxe[CHUNK];
+      bool in[CHUNK], ine[CHUNK];
+      int sp[CHUNK], se[CHUNK];
+#pragma unroll
+      for (int u = 0; u < CHUNK; u++) {
+        const float4 o = offs[e[u] != kNone ? (e[u] >> kImgShift) : kCentre];
+        d[u][0] = qi.x - qj[u].x - o.x;
+        d[u][1] = qi.y - qj[u].y - o.y;
+        d[u][2] = qi.z - qj[u].z - o.z;
+        r2[u] = d[u][0] * d[u][0] + d[u][1] * d[u][1] + d[u][2] * d[u][2];
+        in[u] = (e[u] != kNone) && (r2[u] <= a.cut_p2);
+        ine[u] = in[u] && (r2[u] <= a.cut_e2);
+        rinv[u] = in[u] ? rsqrtf(r2[u]) : 1.f;
+        const float r = in[u] ? r2[u] * rinv[u] : a.phi.r0;
+        sp[u] = spline_locate32_nb(a.phi, r, dxp[u], err);
+        if (PACKED) {
+          se[u] = sp[u];
+          dxe[u] = dxp[u];
+        } else {
+          se[u] = spline_locate32_nb(a.edens, ine[u] ? r : a.edens.r0, dxe[u], err);
+        }
       }
-      const float y = dxe[u];
-      float drho = ce[u].z + y * (2.0f * ce[u].y + 3.0f * y * ce[u].x);
-      if (a.edens.rmult) {
-        const float v = (ce[u].w + y * (ce[u].z + y * (ce[u].y + y * ce[u].x))) * rinv[u];
-        drho = (drho - v) * rinv[u];
+      float4 cp[CHUNK], ce[CHUNK];
+#pragma unroll
+      for (int u = 0; u < CHUNK; u++) {
+        if (PACKED) {
+          cp[u] = a.pack[2 * sp[u]];
+          ce[u] = a.pack[2 * sp[u] + 1];
+        } else {
+          cp[u] = a.phi.c32[sp[u]];
+          ce[u] = a.edens.c32[se[u]];
+        }
       }
-      drho = ine[u] ? drho : 0.f;  // GetRhoDeriv beyond the EDENS range (team.cpp:130)
-      const float fr = in[u] ? -((fpi + fpj[u]) * drho + dp) * rinv[u] : 0.f;
-      fx.add(fr * d[u][0]);
-      fy.add(fr * d[u][1]);
-      fz.add(fr * d[u][2]);
-      if (ENERGY) {
-        pot.add(in[u] ? 0.5f * p : 0.f);
-        vir.add(fr * r2[u]);
+#pragma unroll
+      for (int u = 0; u < CHUNK; u++) {
+        const float x = dxp[u];
+        float p = cp[u].w + x * (cp[u].z + x * (cp[u].y + x * cp[u].x));
+        float dp = cp[u].z + x * (2.0f * cp[u].y + 3.0f * x * cp[u].x);
+        if (a.phi.rmult) {
+          p *= rinv[u];
+          dp = (dp - p) * rinv[u];
+        }
+        const float y = dxe[u];
+        float drho = ce[u].z + y * (2.0f * ce[u].y + 3.0f * y * ce[u].x);
+        if (a.edens.rmult) {
+          const float v = (ce[u].w + y * (ce[u].z + y * (ce[u].y + y * ce[u].x))) * rinv[u];
+          drho = (drho - v) * rinv[u];
+        }
+        drho = ine[u] ? drho : 0.f;  // GetRhoDeriv beyond the EDENS range (team.cpp:130)
+        const float fr = in[u] ? -((fpi + qj[u].w) * drho + dp) * rinv[u] : 0.f;
+        fx.add(fr * d[u][0]);
+        fy.add(fr * d[u][1]);
+        fz.add(fr * d[u][2]);
+        if (ENERGY) {
+          pot.add(in[u] ? 0.5f * p : 0.f);
+          vir.add(fr * r2[u]);
+        }
       }
     }
   }
@@ -255,7 +276,7 @@ void build_pack(Ctx& c, int tp, int te) {
 struct Shape {
   int lanes, chunk;
 };
-constexpr Shape kShapes[] = {{4, 2}, {4, 4}, {8, 1}, {8, 2}, {2, 4}};
+constexpr Shape kShapes[] = {{4, 2}, {4, 4}, {8, 1}, {8, 2}, {2, 4}, {16, 1}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 template <int L, int C>
@@ -285,7 +306,8 @@ void launch(Ctx& c, const EamArgs& a, bool energy, int shape, bool pass1 = true,
     case 1: launch_shape<4, 4>(c, a, energy, pass1, pass2); break;
     case 2: launch_shape<8, 1>(c, a, energy, pass1, pass2); break;
     case 3: launch_shape<8, 2>(c, a, energy, pass1, pass2); break;
-    default: launch_shape<2, 4>(c, a, energy, pass1, pass2); break;
+    case 4: launch_shape<2, 4>(c, a, energy, pass1, pass2); break;
+    default: launch_shape<16, 1>(c, a, energy, pass1, pass2); break;
   }
 }
 
@@ -327,7 +349,7 @@ void eam_forces(Ctx& c, bool energy) {
   a.nbr = c.nbr;
   a.nnbr = c.nnbr;
   a.rho = c.rho;
-  a.fp = c.fp;
+  a.qf = c.qf;
   a.emb = c.emb;
   a.f = c.f32;
   a.pot = c.pot32;

@@ -11,9 +11,9 @@
 //   scatter    counting-sort slots; cells then sorted by original index so the
 //              order is deterministic
 //   permute    gather every per-atom array into cell order (double buffer)
-//   build      one thread per atom scans the 27 stencil cells, tests
+//   build      1-16 lanes per atom scan the 27 stencil cells, test
 //              r^2 <= rlist^2 in FP64 exactly as the reference (shift the
-//              image first, subtract, sum without FMA) and writes the ELL list
+//              image first, subtract, sum without FMA) and write the ELL list
 //              nbr[nidx(i, k)] = slot | stencil-offset << 27 (ELL in blocks
 //              of four, see nidx); a snapshot kernel first writes the FP32
 //              cell-relative positions used by the FP32 force kernels.
@@ -356,6 +356,12 @@ __global__ void __launch_bounds__(256) k_snapshot(BuildArgs a) {
 // distance (error below 1e-5 of rlist^2, see the band) settles every
 // candidate outside a thin band around rlist without touching the FP64
 // coordinates.
+//
+// L lanes serve one atom: in each round the L lanes test L consecutive
+// candidates of a cell and the members are appended in candidate order (a
+// wavefront ballot gives each lane its position), so every L writes the same
+// list in the same order.
+template <int L>
 __global__ void __launch_bounds__(256) k_build(BuildArgs a) {
 #pragma clang fp contract(off)
   __shared__ float4 offs[27];
@@ -365,15 +371,18 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a) {
                           (float)oz * (float)a.g.h[2], 0.f);
   }
   __syncthreads();
-  const int s = xcd_block() * blockDim.x + threadIdx.x;
-  if (s >= a.n) return;
+  const int gid = xcd_block() * blockDim.x + threadIdx.x;
+  const int s = gid / L, j = gid % L;
+  if (s >= a.n) return;  // whole lane groups leave together (L divides 64)
+  const int gbase = (int)(threadIdx.x & 63) - j;
+  const unsigned below = (1u << j) - 1u;
   const int n = a.n;
   const double xi = a.x[s], yi = a.x[(size_t)n + s], zi = a.x[2 * (size_t)n + s];
   const float4 qi = a.q[s];
   int k0, k1, k2;
   snap_cell(qi.w, k0, k1, k2);
   if (a.has_flags && !(a.flag[s] & OMDG_FLAG_ACTIVE)) {
-    a.nnbr[s] = 0;
+    if (j == 0) a.nnbr[s] = 0;
     return;
   }
   int cnt = 0;
@@ -407,27 +416,37 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a) {
         const bool noshift = (sx | sy | sz) == 0;
         const float4 o = offs[img];
         const int e = a.start[cc + 1];
-        for (int t = a.start[cc]; t < e; t++) {
-          if (t == s && noshift) continue;
-          if (a.has_flags && !(a.flag[t] & OMDG_FLAG_ACTIVE)) continue;
-          const float4 qt = a.q[t];
-          const float fx = qi.x - qt.x - o.x, fy = qi.y - qt.y - o.y, fz = qi.z - qt.z - o.z;
-          const float r2f = fx * fx + fy * fy + fz * fz;
-          bool in;
-          if (r2f < a.rl2_lo) {
-            in = true;
-          } else if (r2f > a.rl2_hi) {
-            in = false;
-          } else {
-            const double ddx = xi - shifted(a.x[t], sx, a.g.L[0]);
-            const double ddy = yi - shifted(a.x[(size_t)n + t], sy, a.g.L[1]);
-            const double ddz = zi - shifted(a.x[2 * (size_t)n + t], sz, a.g.L[2]);
-            const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            in = r2 <= a.rl2;
+        for (int t0 = a.start[cc]; t0 < e; t0 += L) {
+          const int t = t0 + j;
+          bool in = false;
+          if (t < e && !(t == s && noshift) &&
+              (!a.has_flags || (a.flag[t] & OMDG_FLAG_ACTIVE))) {
+            const float4 qt = a.q[t];
+            const float fx = qi.x - qt.x - o.x, fy = qi.y - qt.y - o.y, fz = qi.z - qt.z - o.z;
+            const float r2f = fx * fx + fy * fy + fz * fz;
+            if (r2f < a.rl2_lo) {
+              in = true;
+            } else if (r2f <= a.rl2_hi) {
+              const double ddx = xi - shifted(a.x[t], sx, a.g.L[0]);
+              const double ddy = yi - shifted(a.x[(size_t)n + t], sy, a.g.L[1]);
+              const double ddz = zi - shifted(a.x[2 * (size_t)n + t], sz, a.g.L[2]);
+              const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+              in = r2 <= a.rl2;
+            }
           }
-          if (in) {
-            if (cnt < a.maxnbr) a.nbr[nidx(s, cnt, n)] = (uint32_t)t | (img << kImgShift);
-            cnt++;
+          if (L == 1) {
+            if (in) {
+              if (cnt < a.maxnbr) a.nbr[nidx(s, cnt, n)] = (uint32_t)t | (img << kImgShift);
+              cnt++;
+            }
+          } else {
+            const unsigned gm =
+                (unsigned)(__ballot(in) >> gbase) & (unsigned)((1ull << L) - 1ull);
+            if (in) {
+              const int pos = cnt + __popc(gm & below);
+              if (pos < a.maxnbr) a.nbr[nidx(s, pos, n)] = (uint32_t)t | (img << kImgShift);
+            }
+            cnt += __popc(gm);
           }
         }
       }
@@ -435,9 +454,21 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a) {
   }
   // the stored count never exceeds the storage; an overflow is reported and
   // the host rebuilds with a longer list before any force kernel runs
-  a.nnbr[s] = cnt < a.maxnbr ? cnt : a.maxnbr;
-  if (cnt > a.maxnbr) atomicOr(a.stat + ST_ERR, ERR_NBR_OVERFLOW);
-  atomicMax(a.stat + ST_MAXNBR, cnt);
+  if (j == 0) {
+    a.nnbr[s] = cnt < a.maxnbr ? cnt : a.maxnbr;
+    if (cnt > a.maxnbr) atomicOr(a.stat + ST_ERR, ERR_NBR_OVERFLOW);
+    atomicMax(a.stat + ST_MAXNBR, cnt);
+  }
+}
+
+void launch_build(Ctx& c, const BuildArgs& a, int lanes) {
+  const long t = (long)a.n * lanes;
+  switch (lanes) {
+    case 16: k_build<16><<<grid_blocks(t, 256), 256, 0, c.stream>>>(a); break;
+    case 8: k_build<8><<<grid_blocks(t, 256), 256, 0, c.stream>>>(a); break;
+    case 4: k_build<4><<<grid_blocks(t, 256), 256, 0, c.stream>>>(a); break;
+    default: k_build<1><<<grid_blocks(t, 256), 256, 0, c.stream>>>(a); break;
+  }
 }
 
 __global__ void k_reset_build_stats(int* stat) {
@@ -558,7 +589,8 @@ void nbr_build(Ctx& c) {
   a.rl2_hi = (float)(a.rl2 * (1.0 + 3e-5));
   a.g = gd;
   k_snapshot<<<grid_blocks(n, 256), 256, 0, c.stream>>>(a);
-  k_build<<<grid_blocks(n, 256), 256, 0, c.stream>>>(a);
+  if (c.build_lanes < 0) c.build_lanes = pick_build_lanes(c, n, [&](int l) { launch_build(c, a, l); });
+  launch_build(c, a, c.build_lanes);
   OMDG_HIP(hipGetLastError());
   c.forces_valid = false;
 }

@@ -169,7 +169,7 @@ static void free_atoms(Ctx& c) {
   dfree(c.pot64);
   dfree(c.vir64);
   dfree(c.rho);
-  dfree(c.fp);
+  dfree(c.qf);
   dfree(c.emb);
   dfree(c.perm);
   dfree(c.dmax);
@@ -199,7 +199,7 @@ static void alloc_atoms(Ctx& c, int n) {
   dalloc(c.pot64, n);
   dalloc(c.vir64, n);
   dalloc(c.rho, n);
-  dalloc(c.fp, n);
+  dalloc(c.qf, n);
   dalloc(c.emb, n);
   dalloc(c.perm, n);
   dalloc(c.dmax, 2 * (size_t)grid_blocks(n, 256));

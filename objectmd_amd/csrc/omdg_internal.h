@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -141,7 +143,7 @@ struct Ctx {
   double* pot64 = nullptr;
   double* vir64 = nullptr;
   float* rho = nullptr;
-  float* fp = nullptr;  // F'(rho)
+  float4* qf = nullptr; // EAM: snapshot xyz with F'(rho) in w
   float* emb = nullptr; // F(rho)
   float* dmax = nullptr;    // per-block maxima of the drift (2 per block)
   float* bond_b = nullptr;  // Tersoff bond order per near slot (slot-major like near)
@@ -188,6 +190,7 @@ struct Ctx {
   float4* eam_pack = nullptr;
   bool eam_pack_ok = false;
   int eam_shape[2] = {-1, -1};  // tuned (lanes, chunk) shape of the two passes
+  int build_lanes = -1;         // lanes per atom of the list build (tuned on first use)
   int tab_index[3][kMaxTypes][kMaxTypes];  // kind -> (ta,tb) -> table id
   SwParams sw{};
   TersoffParams ters{};
@@ -337,4 +340,36 @@ struct ApiFail {
   std::string msg;
   ApiFail(int c, std::string m) : code(c), msg(std::move(m)) {}
 };
+
+// Lanes per atom of a list build.  Every choice writes the same list in the
+// same order, so systems of 16k atoms and more time 1, 4, 8 and 16 lanes on
+// their first build (one warm-up and two timed launches each) and keep the
+// fastest; smaller ones take 16.
+template <class F>
+int pick_build_lanes(Ctx& c, long n, F&& launch) {
+  if (n < 16384) return 16;
+  hipEvent_t e0, e1;
+  OMDG_HIP(hipEventCreate(&e0));
+  OMDG_HIP(hipEventCreate(&e1));
+  int best = 1;
+  float best_ms = 1e30f;
+  for (int l : {1, 4, 8, 16}) {
+    launch(l);
+    OMDG_HIP(hipEventRecord(e0, c.stream));
+    launch(l);
+    launch(l);
+    OMDG_HIP(hipEventRecord(e1, c.stream));
+    OMDG_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    OMDG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = l;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (std::getenv("OMDG_VERBOSE")) std::fprintf(stderr, "[omdg] list build: %d lanes per atom\n", best);
+  return best;
+}
 }  // namespace omdg

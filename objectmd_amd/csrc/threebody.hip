@@ -78,34 +78,6 @@ __device__ inline float3 disp(const float4& a, const float4& b, const float4& o)
 }
 
 // ---------------------------------------------------------------- SW ------
-struct SwTerm {
-  float gj[3];  // force on leg j
-  float gk[3];  // force on leg k
-  float e;      // triplet energy
-};
-
-// ThreeBodyTerm of sw.cpp:191-236 with d1 = x_j - x_i, d2 = x_k - x_i.
-__device__ inline void sw_three(const TBArgs& a, const float* d1, float r1, const float* d2,
-                                float r2, SwTerm& t) {
-  const float ira0 = 1.0f / (r1 - a.rc), ira1 = 1.0f / (r2 - a.rc);
-  const float fi0 = a.c7 * ira0, fi1 = a.c7 * ira1;
-  const float ir12 = 1.0f / (r1 * r2);
-  const float fex = expf(fi0) * expf(fi1);
-  const float cs = (d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2]) * ir12;
-  const float dc = cs - a.c10;
-  const float ftheta = a.c9 * fex * dc;
-  const float ft12 = ir12 * ftheta;
-  const float fp = a.c8 * fex * dc * dc;
-  const float fcos = cs * ftheta;
-  const float fr0 = fp * fi0 * ira0 / r1, fr1 = fp * fi1 * ira1 / r2;
-  const float ft0 = fcos / (r1 * r1), ft1 = fcos / (r2 * r2);
-  for (int m = 0; m < 3; m++) {
-    t.gj[m] = d1[m] * (fr0 + ft0) - d2[m] * ft12;
-    t.gk[m] = d2[m] * (fr1 + ft1) - d1[m] * ft12;
-  }
-  t.e = fp;
-}
-
 // Two-body SW with the ZBL core and splice; returns fr (force on the first
 // atom = fr * (x_first - x_second)) and the half energy.
 __device__ inline void sw_two(const TBArgs& a, float r, float& fr, float& ehalf) {
@@ -384,44 +356,73 @@ __device__ inline void ters_fc(const TersoffParams& p, float r, float& fc, float
   }
 }
 
-// zeta term T = fc(r_ik) g(theta) exp((l3 (r_ij - r_ik))^3) and its
-// gradients with respect to x_j (gj) and x_k (gk); x_i gets -(gj + gk).
-__device__ inline float ters_zeta_term(const TersoffParams& p, float l3m, const float* dij,
-                                       float rij, const float* dik, float rik, float* gj,
-                                       float* gk) {
-  float fck, dfck;
-  ters_fc(p, rik, fck, dfck);
-  const float c2 = p.c * p.c, d2 = p.d * p.d;
-  const float cs = (dij[0] * dik[0] + dij[1] * dik[1] + dij[2] * dik[2]) / (rij * rik);
-  const float hc = p.h - cs;
-  const float den = d2 + hc * hc;
-  const float g = 1.f + c2 / d2 - c2 / den;
-  const float dg = -2.f * c2 * hc / (den * den);
-  const float dl = rij - rik;
-  const float ex = expf(l3m * dl * dl * dl);
-  const float dex = ex * l3m * 3.f * dl * dl;
-  if (gj) {
-    for (int q = 0; q < 3; q++) {
-      const float uj = dij[q] / rij, uk = dik[q] / rik;
-      const float dcj = (uk - cs * uj) / rij;
-      const float dck = (uj - cs * uk) / rik;
-      gj[q] = fck * dg * dcj * ex + fck * g * dex * uj;
-      gk[q] = dfck * uk * g * ex + fck * dg * dck * ex - fck * g * dex * uk;
-    }
-  }
-  return fck * g * ex;
+// A Tersoff leg: unit vector, length, 1/length and the cutoff with its
+// derivative.
+struct TLeg {
+  float ux, uy, uz, r, ir, fc, dfc;
+};
+
+__device__ inline TLeg tleg(const TersoffParams& p, const float4& d) {
+  TLeg L;
+  L.r = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+  L.ir = rcp(L.r);
+  L.ux = d.x * L.ir;
+  L.uy = d.y * L.ir;
+  L.uz = d.z * L.ir;
+  ters_fc(p, L.r, L.fc, L.dfc);
+  return L;
 }
 
-// Pure pair part of a Tersoff bond: 1/2 dV/dr / r at fixed b and, for the
-// energy, 1/2 V = 1/2 fc (fR + b fA).
-__device__ inline float ters_pair(const TersoffParams& p, float r, float b, float* ehalf) {
-  float fc, dfc;
-  ters_fc(p, r, fc, dfc);
-  const float fR = p.A * expf(-p.lambda1 * r);
-  const float fA = -p.B * expf(-p.lambda2 * r);
-  if (ehalf) *ehalf = 0.5f * fc * (fR + b * fA);
-  const float dV = dfc * (fR + b * fA) + fc * (-p.lambda1 * fR - p.lambda2 * b * fA);
-  return 0.5f * dV / r;
+// Constants of the angular function g(theta) = 1 + c^2/d^2 - c^2/(d^2 + (h - cos)^2).
+struct TCon {
+  float c2, c2d2, d2, h, l3m;
+};
+
+__device__ inline TCon tcon(const TersoffParams& p) {
+  TCon k;
+  k.c2 = p.c * p.c;
+  k.d2 = p.d * p.d;
+  k.c2d2 = k.c2 / k.d2;
+  k.h = p.h;
+  k.l3m = p.lambda3 * p.lambda3 * p.lambda3;
+  return k;
+}
+
+// zeta term T = fc(r_k) g(theta) exp((l3 (r_j - r_k))^3) of bond (i,j) with
+// third atom k, from the legs j and k of centre i.  GRAD 1: g3 = dT/dd_j,
+// GRAD 2: g3 = dT/dd_k (d = x - x_i; the centre gets minus their sum).
+template <int GRAD>
+__device__ inline float ters_t(const TCon& k, const TLeg& j, const TLeg& q, float* g3) {
+  const float cs = j.ux * q.ux + j.uy * q.uy + j.uz * q.uz;
+  const float hc = k.h - cs;
+  const float iden = rcp(k.d2 + hc * hc);
+  const float g = 1.f + k.c2d2 - k.c2 * iden;
+  const float dl = j.r - q.r;
+  const float ex = k.l3m != 0.f ? expf(k.l3m * dl * dl * dl) : 1.f;
+  if (GRAD == 1) {
+    // dcos/dd_j = (u_k - cos u_j) / r_j
+    const float a = q.fc * ex * (-2.f * k.c2 * hc * iden * iden) * j.ir;
+    const float b = q.fc * g * ex * k.l3m * 3.f * dl * dl - a * cs;
+    g3[0] = a * q.ux + b * j.ux;
+    g3[1] = a * q.uy + b * j.uy;
+    g3[2] = a * q.uz + b * j.uz;
+  } else if (GRAD == 2) {
+    // dcos/dd_k = (u_j - cos u_k) / r_k
+    const float a = q.fc * ex * (-2.f * k.c2 * hc * iden * iden) * q.ir;
+    const float b = g * ex * (q.dfc - q.fc * k.l3m * 3.f * dl * dl) - a * cs;
+    g3[0] = a * j.ux + b * q.ux;
+    g3[1] = a * j.uy + b * q.uy;
+    g3[2] = a * j.uz + b * q.uz;
+  }
+  return q.fc * g * ex;
+}
+
+// Pure pair part of a Tersoff bond: 1/2 dV/dr / r at fixed b.
+__device__ inline float ters_pair(const TersoffParams& p, const TLeg& L, float b) {
+  const float fR = p.A * expf(-p.lambda1 * L.r);
+  const float fA = -p.B * expf(-p.lambda2 * L.r);
+  const float dV = L.dfc * (fR + b * fA) + L.fc * (-p.lambda1 * fR - p.lambda2 * b * fA);
+  return 0.5f * dV * L.ir;
 }
 
 // Pass 1: the list scan writes the near list {x_j - x_i, entry} of the
@@ -438,7 +439,6 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_bond(TBArgs a) {
   if (i >= a.n) return;
   const int n = a.n;
   const TersoffParams& p = a.ts;
-  const float l3m = p.lambda3 * p.lambda3 * p.lambda3;
   const float4 qi = a.q[i];
   const int cnt = a.nnbr[i];
   const Col ni{a.near + i, (size_t)n};
@@ -459,21 +459,16 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_bond(TBArgs a) {
   }
   if (nn > kMaxNear) nn = kMaxNear;
   a.near_n[i] = nn;
+  const TCon tk = tcon(p);
   float pot = 0.f;
   for (int u = 0; u < nn; u++) {
-    const float4 du = ni[u];
-    const float dij[3] = {du.x, du.y, du.z};
-    const float rij = sqrtf(du.x * du.x + du.y * du.y + du.z * du.z);
+    const TLeg Lu = tleg(p, ni[u]);
+    const float rij = Lu.r, fc = Lu.fc;
     float zeta = 0.f;
     for (int w = 0; w < nn; w++) {
       if (w == u) continue;
-      const float4 dw = ni[w];
-      const float dik[3] = {dw.x, dw.y, dw.z};
-      const float rik = sqrtf(dw.x * dw.x + dw.y * dw.y + dw.z * dw.z);
-      zeta += ters_zeta_term(p, l3m, dij, rij, dik, rik, nullptr, nullptr);
+      zeta += ters_t<0>(tk, Lu, tleg(p, ni[w]), nullptr);
     }
-    float fc, dfc;
-    ters_fc(p, rij, fc, dfc);
     const float fA = -p.B * expf(-p.lambda2 * rij);
     float b = 1.f, db = 0.f;
     if (zeta > 0.f) {
@@ -519,35 +514,32 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_centre(TBArgs a) {
   if (i >= a.n) return;
   const int n = a.n;
   const TersoffParams& p = a.ts;
-  const float l3m = p.lambda3 * p.lambda3 * p.lambda3;
   const int nn = a.near_n[i];
   const Col ni{a.near + i, (size_t)n}, gi{a.gsum + i, (size_t)n};
+  const TCon tk = tcon(p);
   Kahan fx, fy, fz;
   for (int u = 0; u < nn; u++) {
     const float4 du = ni[u];
-    const float dij[3] = {du.x, du.y, du.z};
-    const float rij = sqrtf(du.x * du.x + du.y * du.y + du.z * du.z);
+    const TLeg Lu = tleg(p, du);
     const float Pu = a.bond_p[(size_t)u * n + i];
-    const float pr = ters_pair(p, rij, a.bond_b[(size_t)u * n + i], nullptr);
-    float sx = pr * dij[0], sy = pr * dij[1], sz = pr * dij[2];
+    const float pr = ters_pair(p, Lu, a.bond_b[(size_t)u * n + i]);
+    float sx = pr * du.x, sy = pr * du.y, sz = pr * du.z;
     for (int w = 0; w < nn; w++) {
       if (w == u) continue;
-      const float4 dw = ni[w];
-      const float dik[3] = {dw.x, dw.y, dw.z};
-      const float rik = sqrtf(dw.x * dw.x + dw.y * dw.y + dw.z * dw.z);
-      float gj[3], gk[3];
+      const TLeg Lw = tleg(p, ni[w]);
+      float g3[3];
       if (Pu != 0.f) {
-        ters_zeta_term(p, l3m, dij, rij, dik, rik, gj, gk);  // bond u, u as the j-atom
-        sx += Pu * gj[0];
-        sy += Pu * gj[1];
-        sz += Pu * gj[2];
+        ters_t<1>(tk, Lu, Lw, g3);  // bond u, u as the j-atom
+        sx += Pu * g3[0];
+        sy += Pu * g3[1];
+        sz += Pu * g3[2];
       }
       const float Pw = a.bond_p[(size_t)w * n + i];
       if (Pw != 0.f) {
-        ters_zeta_term(p, l3m, dik, rik, dij, rij, gj, gk);  // bond w, u as a k-atom
-        sx += Pw * gk[0];
-        sy += Pw * gk[1];
-        sz += Pw * gk[2];
+        ters_t<2>(tk, Lw, Lu, g3);  // bond w, u as a k-atom
+        sx += Pw * g3[0];
+        sy += Pw * g3[1];
+        sz += Pw * g3[2];
       }
     }
     gi[u] = make_float4(-sx, -sy, -sz, 0.f);
