@@ -254,6 +254,7 @@ __global__ void __launch_bounds__(kBlock) k_batch_build(BBArgs a) {
   const double xi = a.x[s], yi = a.x[(size_t)n + s], zi = a.x[2 * (size_t)n + s];
   const float hx = (float)g.h[0], hy = (float)g.h[1], hz = (float)g.h[2];
   int cnt = 0;
+  uint4 blk = make_uint4(0u, 0u, 0u, 0u);
   for (int dz = -1; dz <= 1; dz++) {
     int kz = k2 + dz, sz = 0;
     if ((g.pbc >> 2) & 1) {
@@ -283,34 +284,49 @@ __global__ void __launch_bounds__(kBlock) k_batch_build(BBArgs a) {
         const bool noshift = (sx | sy | sz) == 0;
         const float ox = (float)dx * hx, oy = (float)dy * hy, oz = (float)dz * hz;
         const int e = a.start[cc + 1];
-        for (int u0 = a.start[cc]; u0 < e; u0 += L) {
-          const int u = u0 + j;
-          bool in = false;
-          int t = 0;
-          if (u < e) {
-            const float4 qt = a.qs[u];
-            const uint32_t w = __float_as_uint(qt.w);
-            t = (int)(w & kSlotMask);
-            if (!(t == s && noshift) && (w & kCandActive)) {
-              const int pt = ti * kMaxTypes + (int)((w >> kCandTypeShift) & 3u);
-              const float fx = qi.x - qt.x - ox, fy = qi.y - qt.y - oy, fz = qi.z - qt.z - oz;
-              const float r2f = fx * fx + fy * fy + fz * fz;
-              if (r2f < a.rl2_lo[pt]) {
-                in = true;
-              } else if (r2f <= a.rl2_hi[pt]) {
-                const double ddx = xi - shifted(a.x[t], sx, g.L[0]);
-                const double ddy = yi - shifted(a.x[(size_t)n + t], sy, g.L[1]);
-                const double ddz = zi - shifted(a.x[2 * (size_t)n + t], sz, g.L[2]);
-                in = ddx * ddx + ddy * ddy + ddz * ddz <= a.rl2[pt];
+        // membership of the packed candidate qt (slot t returned)
+        auto member = [&](const float4& qt, int& t) -> bool {
+          const uint32_t w = __float_as_uint(qt.w);
+          t = (int)(w & kSlotMask);
+          if ((t == s && noshift) || !(w & kCandActive)) return false;
+          const int pt = ti * kMaxTypes + (int)((w >> kCandTypeShift) & 3u);
+          const float fx = qi.x - qt.x - ox, fy = qi.y - qt.y - oy, fz = qi.z - qt.z - oz;
+          const float r2f = fx * fx + fy * fy + fz * fz;
+          if (r2f < a.rl2_lo[pt]) return true;
+          if (r2f > a.rl2_hi[pt]) return false;
+          const double ddx = xi - shifted(a.x[t], sx, g.L[0]);
+          const double ddy = yi - shifted(a.x[(size_t)n + t], sy, g.L[1]);
+          const double ddz = zi - shifted(a.x[2 * (size_t)n + t], sz, g.L[2]);
+          return ddx * ddx + ddy * ddy + ddz * ddz <= a.rl2[pt];
+        };
+        if (L == 1) {
+          // four candidates in flight; entries gather in a block of four and
+          // leave as one 16-B store
+          for (int u0 = a.start[cc]; u0 < e; u0 += 4) {
+            float4 qt[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) qt[k] = a.qs[u0 + k < e ? u0 + k : e - 1];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+              int t = 0;
+              if (u0 + k < e && member(qt[k], t)) {
+                const uint32_t ent = (uint32_t)t | (img << kImgShift);
+                const int slot = cnt & 3;
+                blk.x = slot == 0 ? ent : blk.x;
+                blk.y = slot == 1 ? ent : blk.y;
+                blk.z = slot == 2 ? ent : blk.z;
+                blk.w = slot == 3 ? ent : blk.w;
+                cnt++;
+                if (slot == 3 && cnt <= a.maxnbr)
+                  *reinterpret_cast<uint4*>(a.nbr + nidx(s, cnt - 4, n)) = blk;
               }
             }
           }
-          if (L == 1) {
-            if (in) {
-              if (cnt < a.maxnbr) a.nbr[nidx(s, cnt, n)] = (uint32_t)t | (img << kImgShift);
-              cnt++;
-            }
-          } else {
+        } else {
+          for (int u0 = a.start[cc]; u0 < e; u0 += L) {
+            const int u = u0 + j;
+            int t = 0;
+            const bool in = u < e && member(a.qs[u], t);
             const unsigned gm =
                 (unsigned)(__ballot(in) >> gbase) & (unsigned)((1ull << L) - 1ull);
             if (in) {
@@ -323,6 +339,8 @@ __global__ void __launch_bounds__(kBlock) k_batch_build(BBArgs a) {
       }
     }
   }
+  if (L == 1 && (cnt & 3) != 0 && cnt <= a.maxnbr)
+    *reinterpret_cast<uint4*>(a.nbr + nidx(s, cnt & ~3, n)) = blk;  // the partial last block
   if (j == 0) {
     a.nnbr[s] = cnt < a.maxnbr ? cnt : a.maxnbr;
     if (cnt > a.maxnbr) atomicOr(a.stat + ST_ERR, ERR_NBR_OVERFLOW);

@@ -85,7 +85,8 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
   const int s = g / LANES, j = g % LANES;
   const int n = a.n;
   const bool live = s < n;  // no early exit: all lanes take part in the shuffles
-  const float4 qi = live ? a.q[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const int sl = live ? s : 0;  // a valid slot for the unconditional loads
+  const float4 qi = a.q[sl];
   const int nn = live ? a.nnbr[s] : 0;
   int err = 0;
   Kahan rho;
@@ -95,7 +96,9 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
 #pragma unroll
     for (int t = 0; t < kPre; t++) {
       const int k = b0 + t * LANES;
-      ew[t] = (k < nn) ? a.nbr[nidx(s, k, n)] : kNone;
+      // unconditional load (a repeat of the last entry past the end)
+      const uint32_t w = a.nbr[nidx(sl, k < nn ? k : (nn > 0 ? nn - 1 : 0), n)];
+      ew[t] = (k < nn) ? w : kNone;
     }
 #pragma unroll
     for (int t0 = 0; t0 < kPre; t0 += CHUNK) {
@@ -105,7 +108,7 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
       for (int u = 0; u < CHUNK; u++) e[u] = ew[t0 + u];
       float4 qj[CHUNK];
 #pragma unroll
-      for (int u = 0; u < CHUNK; u++) qj[u] = (e[u] != kNone) ? a.q[e[u] & kSlotMask] : qi;
+      for (int u = 0; u < CHUNK; u++) qj[u] = a.q[e[u] != kNone ? (e[u] & kSlotMask) : sl];
       float r[CHUNK], dx[CHUNK];
       bool in[CHUNK];
       int seg[CHUNK];
@@ -117,7 +120,7 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
         const float ez = qi.z - qj[u].z - o.z;
         const float r2 = ex * ex + ey * ey + ez * ez;
         in[u] = (e[u] != kNone) && (r2 <= a.cut_e2);
-        r[u] = in[u] ? r2 * rsqrtf(r2) : a.edens.r0;
+        r[u] = in[u] ? r2 * __builtin_amdgcn_rsqf(r2) : a.edens.r0;
         seg[u] = spline_locate32_nb(a.edens, r[u], dx[u], err);
       }
       float4 c[CHUNK];
@@ -153,7 +156,8 @@ __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
   const int s = g / LANES, j = g % LANES;
   const int n = a.n;
   const bool live = s < n;
-  const float4 qi = live ? a.qf[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const int sl = live ? s : 0;  // a valid slot for the unconditional loads
+  const float4 qi = a.qf[sl];
   const float fpi = qi.w;
   const int nn = live ? a.nnbr[s] : 0;
   int err = 0;
@@ -164,7 +168,9 @@ __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
 #pragma unroll
     for (int t = 0; t < kPre; t++) {
       const int k = b0 + t * LANES;
-      ew[t] = (k < nn) ? a.nbr[nidx(s, k, n)] : kNone;
+      // unconditional load (a repeat of the last entry past the end)
+      const uint32_t w = a.nbr[nidx(sl, k < nn ? k : (nn > 0 ? nn - 1 : 0), n)];
+      ew[t] = (k < nn) ? w : kNone;
     }
 #pragma unroll
     for (int t0 = 0; t0 < kPre; t0 += CHUNK) {
@@ -174,7 +180,7 @@ __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
       for (int u = 0; u < CHUNK; u++) e[u] = ew[t0 + u];
       float4 qj[CHUNK];
 #pragma unroll
-      for (int u = 0; u < CHUNK; u++) qj[u] = (e[u] != kNone) ? a.qf[e[u] & kSlotMask] : qi;
+      for (int u = 0; u < CHUNK; u++) qj[u] = a.qf[e[u] != kNone ? (e[u] & kSlotMask) : sl];
       float d[CHUNK][3], r2[CHUNK], rinv[CHUNK], dxp[CHUNK], dxe[CHUNK];
       bool in[CHUNK], ine[CHUNK];
       int sp[CHUNK], se[CHUNK];
@@ -187,7 +193,7 @@ __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
         r2[u] = d[u][0] * d[u][0] + d[u][1] * d[u][1] + d[u][2] * d[u][2];
         in[u] = (e[u] != kNone) && (r2[u] <= a.cut_p2);
         ine[u] = in[u] && (r2[u] <= a.cut_e2);
-        rinv[u] = in[u] ? rsqrtf(r2[u]) : 1.f;
+        rinv[u] = in[u] ? __builtin_amdgcn_rsqf(r2[u]) : 1.f;
         const float r = in[u] ? r2[u] * rinv[u] : a.phi.r0;
         sp[u] = spline_locate32_nb(a.phi, r, dxp[u], err);
         if (PACKED) {

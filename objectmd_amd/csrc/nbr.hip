@@ -386,6 +386,7 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a) {
     return;
   }
   int cnt = 0;
+  uint4 blk = make_uint4(0u, 0u, 0u, 0u);
   const int pbc = a.g.pbc;
   for (int dz = -1; dz <= 1; dz++) {
     int kz = k2 + dz, sz = 0;
@@ -416,30 +417,46 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a) {
         const bool noshift = (sx | sy | sz) == 0;
         const float4 o = offs[img];
         const int e = a.start[cc + 1];
-        for (int t0 = a.start[cc]; t0 < e; t0 += L) {
-          const int t = t0 + j;
-          bool in = false;
-          if (t < e && !(t == s && noshift) &&
-              (!a.has_flags || (a.flag[t] & OMDG_FLAG_ACTIVE))) {
-            const float4 qt = a.q[t];
-            const float fx = qi.x - qt.x - o.x, fy = qi.y - qt.y - o.y, fz = qi.z - qt.z - o.z;
-            const float r2f = fx * fx + fy * fy + fz * fz;
-            if (r2f < a.rl2_lo) {
-              in = true;
-            } else if (r2f <= a.rl2_hi) {
-              const double ddx = xi - shifted(a.x[t], sx, a.g.L[0]);
-              const double ddy = yi - shifted(a.x[(size_t)n + t], sy, a.g.L[1]);
-              const double ddz = zi - shifted(a.x[2 * (size_t)n + t], sz, a.g.L[2]);
-              const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-              in = r2 <= a.rl2;
+        // membership of candidate slot t with snapshot qt
+        auto member = [&](int t, const float4& qt) -> bool {
+          if ((t == s && noshift) || (a.has_flags && !(a.flag[t] & OMDG_FLAG_ACTIVE)))
+            return false;
+          const float fx = qi.x - qt.x - o.x, fy = qi.y - qt.y - o.y, fz = qi.z - qt.z - o.z;
+          const float r2f = fx * fx + fy * fy + fz * fz;
+          if (r2f < a.rl2_lo) return true;
+          if (r2f > a.rl2_hi) return false;
+          const double ddx = xi - shifted(a.x[t], sx, a.g.L[0]);
+          const double ddy = yi - shifted(a.x[(size_t)n + t], sy, a.g.L[1]);
+          const double ddz = zi - shifted(a.x[2 * (size_t)n + t], sz, a.g.L[2]);
+          return ddx * ddx + ddy * ddy + ddz * ddz <= a.rl2;
+        };
+        if (L == 1) {
+          // four candidate snapshots in flight; entries gather in a block of
+          // four and leave as one 16-B store
+          for (int t0 = a.start[cc]; t0 < e; t0 += 4) {
+            float4 qt[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) qt[u] = a.q[t0 + u < e ? t0 + u : e - 1];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int t = t0 + u;
+              if (t < e && member(t, qt[u])) {
+                const uint32_t ent = (uint32_t)t | (img << kImgShift);
+                const int slot = cnt & 3;
+                blk.x = slot == 0 ? ent : blk.x;
+                blk.y = slot == 1 ? ent : blk.y;
+                blk.z = slot == 2 ? ent : blk.z;
+                blk.w = slot == 3 ? ent : blk.w;
+                cnt++;
+                if (slot == 3 && cnt <= a.maxnbr)
+                  *reinterpret_cast<uint4*>(a.nbr + nidx(s, cnt - 4, n)) = blk;
+              }
             }
           }
-          if (L == 1) {
-            if (in) {
-              if (cnt < a.maxnbr) a.nbr[nidx(s, cnt, n)] = (uint32_t)t | (img << kImgShift);
-              cnt++;
-            }
-          } else {
+        } else {
+          for (int t0 = a.start[cc]; t0 < e; t0 += L) {
+            const int t = t0 + j;
+            const bool in = t < e && member(t, a.q[t]);
             const unsigned gm =
                 (unsigned)(__ballot(in) >> gbase) & (unsigned)((1ull << L) - 1ull);
             if (in) {
@@ -454,6 +471,8 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a) {
   }
   // the stored count never exceeds the storage; an overflow is reported and
   // the host rebuilds with a longer list before any force kernel runs
+  if (L == 1 && (cnt & 3) != 0 && cnt <= a.maxnbr)
+    *reinterpret_cast<uint4*>(a.nbr + nidx(s, cnt & ~3, n)) = blk;  // the partial last block
   if (j == 0) {
     a.nnbr[s] = cnt < a.maxnbr ? cnt : a.maxnbr;
     if (cnt > a.maxnbr) atomicOr(a.stat + ST_ERR, ERR_NBR_OVERFLOW);

@@ -260,6 +260,17 @@ __host__ __device__ inline size_t nidx(int s, int k, int n) {
   return ((size_t)(k >> 2) * (size_t)n + (size_t)s) * 4 + (size_t)(k & 3);
 }
 
+// The four list words of block k0 / 4 of atom s (k0 a multiple of 4) in one
+// 16-B load; words at or past cnt repeat the first, a valid entry, so
+// gathers through them stay in bounds.
+__device__ inline uint4 list_block(const uint32_t* nbr, int s, int k0, int n, int cnt) {
+  uint4 b = *reinterpret_cast<const uint4*>(nbr + nidx(s, k0, n));
+  b.y = k0 + 1 < cnt ? b.y : b.x;
+  b.z = k0 + 2 < cnt ? b.z : b.x;
+  b.w = k0 + 3 < cnt ? b.w : b.x;
+  return b;
+}
+
 // The w word of an FP32 snapshot holds the atom type (2 bits) and the cell
 // coordinates of the last build (10 bits each); it is only ever moved and
 // reinterpreted, never used in float arithmetic.
@@ -344,9 +355,14 @@ struct ApiFail {
 // Lanes per atom of a list build.  Every choice writes the same list in the
 // same order, so systems of 16k atoms and more time 1, 4, 8 and 16 lanes on
 // their first build (one warm-up and two timed launches each) and keep the
-// fastest; smaller ones take 16.
+// fastest; smaller ones take 16.  OMDG_BUILD_LANES (1, 4, 8 or 16) fixes the
+// choice.
 template <class F>
 int pick_build_lanes(Ctx& c, long n, F&& launch) {
+  if (const char* env = std::getenv("OMDG_BUILD_LANES")) {
+    const int l = std::atoi(env);
+    if (l == 1 || l == 4 || l == 8 || l == 16) return l;
+  }
   if (n < 16384) return 16;
   hipEvent_t e0, e1;
   OMDG_HIP(hipEventCreate(&e0));

@@ -38,6 +38,8 @@ constexpr int NR = kSgRadial, NA = kSgAngular, ND = kSgDesc, NH = kSgHidden;
 constexpr int kNear = 24;  // neighbours inside rc kept per atom (shares the SW arrays)
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 struct SgArgs {
@@ -191,19 +193,11 @@ __global__ void k_sg_decide(SgArgs a) {
 }
 
 // ---- pass B: the MLP on the matrix cores ------------------------------------
-__device__ inline short to_bf16(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);  // nearest even
-  return (short)(u >> 16);
-}
-
+// four FP32 values to BF16 (round to nearest even: v_cvt_pk_bf16_f32)
 __device__ inline bf16x4 pack(float a, float b, float c, float d) {
-  bf16x4 r;
-  r[0] = to_bf16(a);
-  r[1] = to_bf16(b);
-  r[2] = to_bf16(c);
-  r[3] = to_bf16(d);
-  return r;
+  const bf16x2 lo = {(__bf16)a, (__bf16)b}, hi = {(__bf16)c, (__bf16)d};
+  const u32x2 u = {__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi)};
+  return __builtin_bit_cast(bf16x4, u);
 }
 
 // A operand of rows [r0, r0+16) and columns [k0, k0+16) of a row-major bf16
@@ -218,7 +212,9 @@ __device__ inline bf16x4 frag(const uint16_t* m, int ld, int r0, int k0, int lan
   return r;
 }
 
-__device__ inline float sigm(float x) { return 1.f / (1.f + expf(-x)); }
+// logistic function with the hardware exponential and reciprocal (a few ulp;
+// the values feed BF16 operands)
+__device__ inline float sigm(float x) { return __builtin_amdgcn_rcpf(1.f + __expf(-x)); }
 
 __global__ void __launch_bounds__(kBlock) k_sg_mlp(SgArgs a) {
   if (a.state[0]) return;  // this step falls back to Tersoff
@@ -248,7 +244,7 @@ __global__ void __launch_bounds__(kBlock) k_sg_mlp(SgArgs a) {
                     (gv.z - v[kMean + g0 + 2]) * v[kIstd + g0 + 2],
                     (gv.w - v[kMean + g0 + 3]) * v[kIstd + g0 + 3]);
     }
-    // layer 1
+    // layer 1 (a1 keeps SiLU' of its outputs)
     f32x4 a1[4];
     bf16x4 bh1[4];
 #pragma unroll
@@ -262,8 +258,9 @@ __global__ void __launch_bounds__(kBlock) k_sg_mlp(SgArgs a) {
 #pragma unroll
       for (int it = 0; it < 4; it++) {
         const float z = acc[it] + v[kB1 + ob * 16 + 4 * grp + it];
-        a1[ob][it] = z;
-        h[it] = z * sigm(z);
+        const float sg = sigm(z);
+        a1[ob][it] = sg * (1.f + z * (1.f - sg));  // SiLU'(z) for the backward pass
+        h[it] = z * sg;
       }
       bh1[ob] = pack(h[0], h[1], h[2], h[3]);
     }
@@ -303,14 +300,8 @@ __global__ void __launch_bounds__(kBlock) k_sg_mlp(SgArgs a) {
       for (int kb = 0; kb < 4; kb++)
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag(w + kW2T, NH, ob * 16, kb * 16, lane),
                                                         bd2[kb], acc, 0, 0, 0);
-      float d[4];
-#pragma unroll
-      for (int it = 0; it < 4; it++) {
-        const float z = a1[ob][it];
-        const float s = sigm(z);
-        d[it] = acc[it] * s * (1.f + z * (1.f - s));
-      }
-      bd1[ob] = pack(d[0], d[1], d[2], d[3]);
+      bd1[ob] = pack(acc[0] * a1[ob][0], acc[1] * a1[ob][1], acc[2] * a1[ob][2],
+                     acc[3] * a1[ob][3]);
     }
     // back through layer 1: dE/dz, then dE/dG = dE/dz / std
 #pragma unroll

@@ -172,37 +172,47 @@ __global__ void __launch_bounds__(kBlock) k_sw_pair(TBArgs a) {
   float pot = 0.f, vir = 0.f;
   int nn = 0;
   const int cnt = a.nnbr[l];
-  for (int k = 0; k < cnt; k++) {
-    const uint32_t e = a.nbr[nidx(l, k, n)];
-    const float4 qj = a.q[e & kSlotMask];
-    const float3 d = disp(ql, qj, off_of(e >> kImgShift, hh));  // x_j - x_l
-    const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
-    if (r2 < 1e-6f) {
-      atomicOr(a.err, ERR_OVERLAP);
-      continue;
-    }
-    const int tj = snap_type(qj.w);
-    const bool pair_sw = l_sw && (a.sw_type < 0 || tj == a.sw_type);
-    if (pair_sw) {
-      if (r2 < a.rc2) {
-        if (nn < kMaxNear) nl[nn] = make_float4(d.x, d.y, d.z, __uint_as_float(e));
-        else atomicOr(a.err, ERR_NEAR);
-        nn++;
+  for (int k0 = 0; k0 < cnt; k0 += 4) {
+    // a block of four list words, then their four snapshots, in flight together
+    const uint4 eb = list_block(a.nbr, l, k0, n, cnt);
+    const uint32_t ev[4] = {eb.x, eb.y, eb.z, eb.w};
+    float4 qv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) qv[u] = a.q[ev[u] & kSlotMask];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (k0 + u >= cnt) break;
+      const uint32_t e = ev[u];
+      const float4 qj = qv[u];
+      const float3 d = disp(ql, qj, off_of(e >> kImgShift, hh));  // x_j - x_l
+      const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+      if (r2 < 1e-6f) {
+        atomicOr(a.err, ERR_OVERLAP);
+        continue;
       }
-    } else {
-      const int slot = tl * kMaxTypes + tj;
-      const int tb = a.tab[slot];
-      if (tb >= 0 && r2 < a.tcut2[slot]) {
-        const float r = sqrtf(r2);
-        float p, dp;
-        spline_read32(a.tabs[tb], r, p, dp, a.err);
-        const float ff = -dp / r;  // force on l = ff * (x_l - x_j)
-        fx.add(-ff * d.x);
-        fy.add(-ff * d.y);
-        fz.add(-ff * d.z);
-        if (ENERGY) {
-          pot += 0.5f * p;
-          vir += ff * r2;
+      const int tj = snap_type(qj.w);
+      const bool pair_sw = l_sw && (a.sw_type < 0 || tj == a.sw_type);
+      if (pair_sw) {
+        if (r2 < a.rc2) {
+          if (nn < kMaxNear) nl[nn] = make_float4(d.x, d.y, d.z, __uint_as_float(e));
+          else atomicOr(a.err, ERR_NEAR);
+          nn++;
+        }
+      } else {
+        const int slot = tl * kMaxTypes + tj;
+        const int tb = a.tab[slot];
+        if (tb >= 0 && r2 < a.tcut2[slot]) {
+          const float r = sqrtf(r2);
+          float p, dp;
+          spline_read32(a.tabs[tb], r, p, dp, a.err);
+          const float ff = -dp / r;  // force on l = ff * (x_l - x_j)
+          fx.add(-ff * d.x);
+          fy.add(-ff * d.y);
+          fz.add(-ff * d.z);
+          if (ENERGY) {
+            pot += 0.5f * p;
+            vir += ff * r2;
+          }
         }
       }
     }
@@ -443,18 +453,27 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_bond(TBArgs a) {
   const int cnt = a.nnbr[i];
   const Col ni{a.near + i, (size_t)n};
   int nn = 0;
-  for (int k = 0; k < cnt; k++) {
-    const uint32_t e = a.nbr[nidx(i, k, n)];
-    const float3 d = disp(qi, a.q[e & kSlotMask], offs[e >> kImgShift]);
-    const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
-    if (r2 < 1e-6f) {
-      atomicOr(a.err, ERR_OVERLAP);
-      continue;
-    }
-    if (r2 < a.rc2) {
-      if (nn < kMaxNear) ni[nn] = make_float4(d.x, d.y, d.z, __uint_as_float(e));
-      else atomicOr(a.err, ERR_NEAR);
-      nn++;
+  for (int k0 = 0; k0 < cnt; k0 += 4) {
+    const uint4 eb = list_block(a.nbr, i, k0, n, cnt);
+    const uint32_t ev[4] = {eb.x, eb.y, eb.z, eb.w};
+    float4 qv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) qv[u] = a.q[ev[u] & kSlotMask];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (k0 + u >= cnt) break;
+      const uint32_t e = ev[u];
+      const float3 d = disp(qi, qv[u], offs[e >> kImgShift]);
+      const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+      if (r2 < 1e-6f) {
+        atomicOr(a.err, ERR_OVERLAP);
+        continue;
+      }
+      if (r2 < a.rc2) {
+        if (nn < kMaxNear) ni[nn] = make_float4(d.x, d.y, d.z, __uint_as_float(e));
+        else atomicOr(a.err, ERR_NEAR);
+        nn++;
+      }
     }
   }
   if (nn > kMaxNear) nn = kMaxNear;

@@ -65,9 +65,12 @@ def wrap_rep(x, lo, hi):
     return O.wrap(x, lo, hi - lo, 3)
 
 
-def test_batch_lists_bit_exact():
+@pytest.mark.parametrize("lanes", ["auto", "1", "16"])
+def test_batch_lists_bit_exact(lanes, monkeypatch):
+    if lanes != "auto":
+        monkeypatch.setenv("OMDG_BUILD_LANES", lanes)
     b = small_batch(11, height=2.5)
-    say(f"[batch-list] n={len(b['x'])} nrep={len(b['off']) - 1}")
+    say(f"[batch-list] n={len(b['x'])} nrep={len(b['off']) - 1} lanes={lanes}")
     eng = engine(b)
     eng.neighbor_build()
     cnt, code = eng.neighbor_export(128)

@@ -3,6 +3,7 @@
 Bars: neighbour lists bit-exact after per-atom sort; FP32-force paths within
 1e-5 relative RMS force error; FP64 pair path within 1e-12."""
 import sys
+import zlib
 
 import numpy as np
 import pytest
@@ -56,7 +57,7 @@ def _lj_engine(s, fp64=True, skin=0.3):
 # neighbour lists
 # ---------------------------------------------------------------------------
 def _list_case(name):
-    rng = np.random.default_rng(hash(name) % 1000)
+    rng = np.random.default_rng(zlib.crc32(name.encode()) % 1000)
     if name == "lj_fcc":
         s = lj_system(cells=10)
         x = wrap_np(s["x"] + rng.normal(0, 0.1, s["x"].shape), s["lo"], s["L"], 7)
@@ -82,11 +83,15 @@ def _list_case(name):
     raise KeyError(name)
 
 
+@pytest.mark.parametrize("lanes", ["auto", "1", "8"])
 @pytest.mark.parametrize("name", ["lj_fcc", "eam_fcc", "gas_pbc", "gas_xy", "gas_free",
                                   "small_box", "slab_xy"])
-def test_neighbor_list_bit_exact(name):
+def test_neighbor_list_bit_exact(name, lanes, monkeypatch):
+    """Every lane count of the build (OMDG_BUILD_LANES) writes the oracle's list."""
+    if lanes != "auto":
+        monkeypatch.setenv("OMDG_BUILD_LANES", lanes)
     x, lo, hi, pbc, rlist = _list_case(name)
-    say(f"[neighbor] case={name} n={len(x)} pbc={pbc} rlist={rlist}")
+    say(f"[neighbor] case={name} n={len(x)} pbc={pbc} rlist={rlist} lanes={lanes}")
     cnt_o, code_o = O.neighbors(x, lo, hi - lo, pbc, rlist, maxn=160)
     eng = om.Engine()
     eng.set_box(lo, hi, pbc)
