@@ -89,6 +89,7 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
   const float4 qi = a.q[sl];
   const int nn = live ? a.nnbr[s] : 0;
   int err = 0;
+  QRange qe;
   Kahan rho;
   for (int b0 = j; b0 < nn; b0 += LANES * kPre) {
     // all list words of this stretch in flight at once
@@ -121,7 +122,7 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
         const float r2 = ex * ex + ey * ey + ez * ez;
         in[u] = (e[u] != kNone) && (r2 <= a.cut_e2);
         r[u] = in[u] ? r2 * __builtin_amdgcn_rsqf(r2) : a.edens.r0;
-        seg[u] = spline_locate32_nb(a.edens, r[u], dx[u], err);
+        seg[u] = spline_locate32_q(a.edens, r[u], dx[u], qe);
       }
       float4 c[CHUNK];
 #pragma unroll
@@ -134,6 +135,7 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
       }
     }
   }
+  err |= range_flags(a.edens, qe);
   if (err) atomicOr(a.err, err);
   const double rs = lanes_sum<LANES>(rho);
   if (live && j == 0) {
@@ -161,6 +163,7 @@ __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
   const float fpi = qi.w;
   const int nn = live ? a.nnbr[s] : 0;
   int err = 0;
+  QRange qp, qe;
   Kahan fx, fy, fz, pot, vir;
   for (int b0 = j; b0 < nn; b0 += LANES * kPre) {
     // all list words of this stretch in flight at once
@@ -195,12 +198,12 @@ __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
         ine[u] = in[u] && (r2[u] <= a.cut_e2);
         rinv[u] = in[u] ? __builtin_amdgcn_rsqf(r2[u]) : 1.f;
         const float r = in[u] ? r2[u] * rinv[u] : a.phi.r0;
-        sp[u] = spline_locate32_nb(a.phi, r, dxp[u], err);
+        sp[u] = spline_locate32_q(a.phi, r, dxp[u], qp);
         if (PACKED) {
           se[u] = sp[u];
           dxe[u] = dxp[u];
         } else {
-          se[u] = spline_locate32_nb(a.edens, ine[u] ? r : a.edens.r0, dxe[u], err);
+          se[u] = spline_locate32_q(a.edens, ine[u] ? r : a.edens.r0, dxe[u], qe);
         }
       }
       float4 cp[CHUNK], ce[CHUNK];
@@ -241,6 +244,8 @@ __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
       }
     }
   }
+  err |= range_flags(a.phi, qp);
+  if (!PACKED) err |= range_flags(a.edens, qe);
   if (err) atomicOr(a.err, err);
   const double sx = lanes_sum<LANES>(fx), sy = lanes_sum<LANES>(fy), sz = lanes_sum<LANES>(fz);
   double sp = 0.0, sv = 0.0;

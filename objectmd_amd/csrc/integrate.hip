@@ -112,8 +112,10 @@ __global__ void __launch_bounds__(kBlock) k_status_max(const float* part, int nb
 }
 
 // FP64 updates are rounded as written (no fused multiply-add), like the
-// reference's x86-64 build.
-template <bool FP64F, bool FLAGS, bool BATCH>
+// reference's x86-64 build.  KICK2 first applies the closing half kick of the
+// previous step (systems without modifies): the same two additions in the
+// same order as the separate kick kernel, one pass over v and f fewer.
+template <bool FP64F, bool FLAGS, bool BATCH, bool KICK2 = false>
 __global__ void __launch_bounds__(kBlock) k_kick_drift(IntArgs a) {
 #pragma clang fp contract(off)
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -141,9 +143,14 @@ __global__ void __launch_bounds__(kBlock) k_kick_drift(IntArgs a) {
       if (move) {
         double v;
         if (FP64F) {
-          v = a.v[k] + hdt * a.f64[k] / m;  // integrator.cpp:236 order
+          v = a.v[k];
+          if (KICK2) v = v + hdt * a.f64[k] / m;  // integrator.cpp:253 order
+          v = v + hdt * a.f64[k] / m;             // integrator.cpp:236 order
         } else {
-          v = a.v[k] + (double)a.f32[k] * hdt_m;
+          const double dv = (double)a.f32[k] * hdt_m;
+          v = a.v[k];
+          if (KICK2) v = v + dv;
+          v = v + dv;
         }
         x = x + dt * v;
         a.v[k] = v;
@@ -329,9 +336,10 @@ IntArgs make_args(Ctx& c, double dt) {
 }
 
 template <bool FP64F, bool FLAGS, bool BATCH>
-void launch_int(Ctx& c, const IntArgs& a, bool drift) {
+void launch_int(Ctx& c, const IntArgs& a, bool drift, bool kick2 = false) {
   const int nb = grid_blocks(c.n, kBlock);
-  if (drift) k_kick_drift<FP64F, FLAGS, BATCH><<<nb, kBlock, 0, c.stream>>>(a);
+  if (drift && kick2) k_kick_drift<FP64F, FLAGS, BATCH, true><<<nb, kBlock, 0, c.stream>>>(a);
+  else if (drift) k_kick_drift<FP64F, FLAGS, BATCH><<<nb, kBlock, 0, c.stream>>>(a);
   else k_kick<FP64F, FLAGS, BATCH><<<nb, kBlock, 0, c.stream>>>(a);
 }
 
@@ -438,6 +446,16 @@ __global__ void __launch_bounds__(kBlock) k_energy_final(const double* part, int
 }  // namespace
 
 void integrate_kick_drift(Ctx& c, double dt) { dispatch_int(c, dt, true); }
+
+bool integrate_can_fuse(const Ctx& c) { return c.nrep == 0 && !c.has_flags; }
+
+void integrate_kick_kick_drift(Ctx& c, double dt) {
+  if (!integrate_can_fuse(c)) throw ApiFail(OMDG_EINVAL, "fused kick needs a system without modifies");
+  IntArgs a = make_args(c, dt);
+  if (c.fp64) launch_int<true, false, false>(c, a, true, true);
+  else launch_int<false, false, false>(c, a, true, true);
+  OMDG_HIP(hipGetLastError());
+}
 
 void integrate_kick(Ctx& c, double dt) { dispatch_int(c, dt, false); }
 

@@ -987,10 +987,14 @@ int omdg_step(omdg_t h, int nsteps, double dt, double* ekin, double* epot) {
     // rebuilds if one more step could cross skin/2, and otherwise schedules
     // the next check before the fastest atom, at 1.5x its speed, could.
     int until_check = 1;
+    // Without modifies the closing half kick of a step joins the opening
+    // kick and drift of the next (same additions, one pass fewer).
+    const bool fuse = integrate_can_fuse(c);
     for (int s = 0; s < nsteps; s++) {
       {
         TimerScope ts(c, TG_DRIFT);
-        integrate_kick_drift(c, dt);
+        if (fuse && s > 0) integrate_kick_kick_drift(c, dt);
+        else integrate_kick_drift(c, dt);
       }
       if (--until_check == 0) {
         integrate_status(c);
@@ -1008,8 +1012,10 @@ int omdg_step(omdg_t h, int nsteps, double dt, double* ekin, double* epot) {
       }
       const bool last = (s == nsteps - 1);
       compute_forces(c, last);
-      TimerScope ts(c, TG_KICK);
-      integrate_kick(c, dt);
+      if (!fuse || last) {
+        TimerScope ts(c, TG_KICK);
+        integrate_kick(c, dt);
+      }
     }
     {
       TimerScope ts(c, TG_REDUCE);

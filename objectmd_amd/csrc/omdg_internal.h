@@ -312,6 +312,10 @@ void surrogate_forces(Ctx& c, bool energy);
 void surrogate_free(Ctx& c);
 void integrate_kick_drift(Ctx& c, double dt);
 void integrate_kick(Ctx& c, double dt);
+// the closing half kick of one step and the opening kick + drift of the next
+// in one pass (single systems without modifies)
+bool integrate_can_fuse(const Ctx& c);
+void integrate_kick_kick_drift(Ctx& c, double dt);
 // the POST_FORCE and POST_INTEGRATION modifies of MDSystem::FirstRun
 // (system.cpp:860-873) without moving anything
 void integrate_first(Ctx& c);

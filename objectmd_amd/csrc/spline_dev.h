@@ -137,6 +137,38 @@ __device__ inline int spline_locate32_nb(const TabView& t, float r, float& dx, i
   return x;
 }
 
+// Extremes of the scaled table coordinate (r - r0) / dr seen by one lane.
+// Range errors are raised once from them (range_flags) rather than per
+// lookup; the bounds are those of spline_locate32_nb.
+struct QRange {
+  float lo = 3.0e38f, hi = -3.0e38f;
+  __device__ inline void add(float q) {
+    lo = fminf(lo, q);
+    hi = fmaxf(hi, q);
+  }
+};
+
+__device__ inline int range_flags(const TabView& t, const QRange& q) {
+  return (q.hi >= (float)t.n ? ERR_RANGE_HI : 0) | (q.lo <= -2.0f ? ERR_RANGE_LO : 0);
+}
+
+// spline_locate32_nb with the range check deferred to range_flags; the
+// argument must not be NaN (callers pass r0 for pairs outside the cut).
+__device__ inline int spline_locate32_q(const TabView& t, float r, float& dx, QRange& qr) {
+  const float rval = r - t.r0;
+  float qf = rval * t.inv_dr;
+  qr.add(qf);
+  qf = fminf(fmaxf(qf, -1.5f), (float)t.n - 0.5f);
+  int x = (int)qf;  // truncation, in [-1, n-1]
+  float d = rval - (float)x * t.dr;
+  const bool top = (x == t.n - 1);
+  d = top ? d + t.dr : d;
+  x = top ? x - 1 : x;
+  x = x < 0 ? 0 : x;
+  dx = d;
+  return x;
+}
+
 // Compensated (Kahan) accumulation of FP32 terms.
 struct Kahan {
   float s = 0.f, c = 0.f;
