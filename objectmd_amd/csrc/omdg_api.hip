@@ -889,11 +889,12 @@ int omdg_neighbor_export(omdg_t h, int maxn, int* cnt, int* code) {
     if (!c.list_valid) throw ApiFail(OMDG_EINVAL, "no neighbour list built");
     OMDG_HIP(hipStreamSynchronize(c.stream));
     const int n = c.n;
-    std::vector<int> id(n), nn(n), cell(n);
+    std::vector<int> id(n), nn(n);
+    std::vector<float4> snap(n);  // w holds the cell of the last build
     std::vector<uint32_t> lst((size_t)c.maxnbr * n);
     OMDG_HIP(hipMemcpy(id.data(), c.id[c.cur], sizeof(int) * n, hipMemcpyDeviceToHost));
     OMDG_HIP(hipMemcpy(nn.data(), c.nnbr, sizeof(int) * n, hipMemcpyDeviceToHost));
-    OMDG_HIP(hipMemcpy(cell.data(), c.cellid, sizeof(int) * n, hipMemcpyDeviceToHost));
+    OMDG_HIP(hipMemcpy(snap.data(), c.qb, sizeof(float4) * n, hipMemcpyDeviceToHost));
     OMDG_HIP(hipMemcpy(lst.data(), c.nbr, sizeof(uint32_t) * lst.size(), hipMemcpyDeviceToHost));
     std::vector<int> rep;
     if (c.nrep > 0) {
@@ -905,14 +906,15 @@ int omdg_neighbor_export(omdg_t h, int maxn, int* cnt, int* code) {
       int i = id[s];
       // the cell grid of the atom's system (its replica's with replicas)
       Grid g = c.grid;
-      int cl = cell[s];
       if (c.nrep > 0) {
         const RepGrid& rg = c.rep_grid_h[rep[s]];
         for (int d = 0; d < 3; d++) g.M[d] = rg.M[d];
         g.pbc = rg.pbc;
-        cl -= (int)rg.cell_off;
       }
-      int k[3] = {cl % g.M[0], (cl / g.M[0]) % g.M[1], cl / (g.M[0] * g.M[1])};
+      uint32_t wb;
+      std::memcpy(&wb, &snap[s].w, sizeof wb);
+      const int k[3] = {(int)((wb >> 2) & 1023u), (int)((wb >> 12) & 1023u),
+                        (int)((wb >> 22) & 1023u)};
       std::vector<int> row;
       row.reserve(nn[s]);
       for (int e = 0; e < nn[s]; e++) {
