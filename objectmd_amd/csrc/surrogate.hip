@@ -97,7 +97,7 @@ __device__ inline float ipow(float a, int k) {
 __device__ inline float ang(const SgArgs& a, int m, float cs, float& dA) {
   if (a.kind == 1) {
     const float t = cs - a.ap[m];
-    const float g = expf(-a.aq[m] * t * t);
+    const float g = __expf(-a.aq[m] * t * t);
     dA = -2.f * a.aq[m] * t * g;
     return g;
   }
@@ -109,8 +109,8 @@ __device__ inline float ang(const SgArgs& a, int m, float cs, float& dA) {
 
 __device__ inline void cutoff(const SgArgs& a, float r, float& fc, float& dfc) {
   const float t = 3.14159265358979f * r / a.rc;
-  fc = 0.5f * (cosf(t) + 1.f);
-  dfc = -0.5f * 3.14159265358979f / a.rc * sinf(t);
+  fc = 0.5f * (__cosf(t) + 1.f);  // t in [0, pi]: the hardware sine and cosine
+  dfc = -0.5f * 3.14159265358979f / a.rc * __sinf(t);
 }
 
 // ---- pass A -----------------------------------------------------------------
@@ -126,20 +126,30 @@ __global__ void __launch_bounds__(kBlock) k_sg_desc(SgArgs a) {
     for (int g = 0; g < ND; g++) G[g] = 0.f;
     int nn = 0;
     const int cnt = a.nnbr[l];
-    for (int k = 0; k < cnt; k++) {
-      const uint32_t e = a.nbr[nidx(l, k, n)];
-      const float4 qj = a.q[e & kSlotMask];
-      const float4 o = off_of(e >> kImgShift, hh);
-      const float dx = qj.x - ql.x + o.x, dy = qj.y - ql.y + o.y, dz = qj.z - ql.z + o.z;
-      const float r2 = dx * dx + dy * dy + dz * dz;
-      if (!(r2 < a.rc2)) continue;
-      if (r2 < 1e-6f) {
-        atomicOr(a.err, ERR_OVERLAP);
-        continue;
+    for (int k0 = 0; k0 < cnt; k0 += 4) {
+      // a block of four list words, then their four snapshots, in flight together
+      const uint4 eb = list_block(a.nbr, l, k0, n, cnt);
+      const uint32_t ev[4] = {eb.x, eb.y, eb.z, eb.w};
+      float4 qv[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) qv[u] = a.q[ev[u] & kSlotMask];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (k0 + u >= cnt) break;
+        const uint32_t e = ev[u];
+        const float4 o = off_of(e >> kImgShift, hh);
+        const float dx = qv[u].x - ql.x + o.x, dy = qv[u].y - ql.y + o.y,
+                    dz = qv[u].z - ql.z + o.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (!(r2 < a.rc2)) continue;
+        if (r2 < 1e-6f) {
+          atomicOr(a.err, ERR_OVERLAP);
+          continue;
+        }
+        if (nn < kNear) a.near[(size_t)nn * n + l] = make_float4(dx, dy, dz, __uint_as_float(e));
+        else atomicOr(a.err, ERR_NEAR);
+        nn++;
       }
-      if (nn < kNear) a.near[(size_t)nn * n + l] = make_float4(dx, dy, dz, __uint_as_float(e));
-      else atomicOr(a.err, ERR_NEAR);
-      nn++;
     }
     if (nn > kNear) nn = kNear;
     for (int u = 0; u < nn; u++) {
@@ -150,15 +160,15 @@ __global__ void __launch_bounds__(kBlock) k_sg_desc(SgArgs a) {
 #pragma unroll
       for (int k = 0; k < NR; k++) {
         const float t = ru - (a.mu0 + a.dmu * (float)k);
-        G[k] += expf(-a.eta_r * t * t) * fu;
+        G[k] += __expf(-a.eta_r * t * t) * fu;
       }
       for (int w = u + 1; w < nn; w++) {
         const float4 dw = a.near[(size_t)w * n + l];
         const float rw = sqrtf(dw.x * dw.x + dw.y * dw.y + dw.z * dw.z);
         float fw, dfw;
         cutoff(a, rw, fw, dfw);
-        const float cs = (du.x * dw.x + du.y * dw.y + du.z * dw.z) / (ru * rw);
-        const float rad = expf(-a.eta_a * (ru * ru + rw * rw)) * fu * fw;
+        const float cs = (du.x * dw.x + du.y * dw.y + du.z * dw.z) * __builtin_amdgcn_rcpf(ru * rw);
+        const float rad = __expf(-a.eta_a * (ru * ru + rw * rw)) * fu * fw;
 #pragma unroll
         for (int m = 0; m < NA; m++) {
           float dA;
@@ -341,7 +351,7 @@ __global__ void __launch_bounds__(kBlock) k_sg_centre(SgArgs a) {
   for (int u = 0; u < nn; u++) {
     const float4 du = a.near[(size_t)u * n + i];
     const float ru = sqrtf(du.x * du.x + du.y * du.y + du.z * du.z);
-    const float iru = 1.f / ru;
+    const float iru = __builtin_amdgcn_rcpf(ru);
     float fu, dfu;
     cutoff(a, ru, fu, dfu);
     // radial: sum_k dE/dG_k phi_k'(r) / r
@@ -349,7 +359,7 @@ __global__ void __launch_bounds__(kBlock) k_sg_centre(SgArgs a) {
 #pragma unroll
     for (int k = 0; k < NR; k++) {
       const float t = ru - (a.mu0 + a.dmu * (float)k);
-      const float g = expf(-a.eta_r * t * t);
+      const float g = __expf(-a.eta_r * t * t);
       coef += dg[k] * g * (-2.f * a.eta_r * t * fu + dfu);
     }
     coef *= iru;
@@ -361,9 +371,9 @@ __global__ void __launch_bounds__(kBlock) k_sg_centre(SgArgs a) {
       const float rw = sqrtf(dw.x * dw.x + dw.y * dw.y + dw.z * dw.z);
       float fw, dfw;
       cutoff(a, rw, fw, dfw);
-      const float irw = 1.f / rw;
+      const float irw = __builtin_amdgcn_rcpf(rw);
       const float cs = (du.x * dw.x + du.y * dw.y + du.z * dw.z) * iru * irw;
-      const float ea = expf(-a.eta_a * (ru * ru + rw * rw));
+      const float ea = __expf(-a.eta_a * (ru * ru + rw * rw));
       const float rad = ea * fu * fw;
       const float drad = -2.f * a.eta_a * rad + ea * dfu * fw * iru;  // times d_u
       float sc = 0.f, sr = 0.f;  // coefficients of d cos/d x_u and of d_u
