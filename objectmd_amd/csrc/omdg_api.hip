@@ -481,6 +481,7 @@ int omdg_destroy(omdg_t h) {
   dfree(c.bond_b);
   dfree(c.bond_p);
   dfree(c.near);
+  dfree(c.nid);
   dfree(c.gsum);
   dfree(c.near_n);
   dfree(c.cell_start);

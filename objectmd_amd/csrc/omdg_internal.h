@@ -149,6 +149,7 @@ struct Ctx {
   float* bond_b = nullptr;  // Tersoff bond order per near slot (slot-major like near)
   float* bond_p = nullptr;  // Tersoff zeta prefactor per near slot
   size_t cap_bond = 0;
+  uint4* nid = nullptr;     // list words of each atom's first four near entries
   float4* near = nullptr;   // SW near lists (displacement, entry) per slot
   float4* gsum = nullptr;   // SW per-leg three-body force and energy share
   int* near_n = nullptr;
@@ -269,6 +270,43 @@ __device__ inline uint4 list_block(const uint32_t* nbr, int s, int k0, int n, in
   b.z = k0 + 2 < cnt ? b.z : b.x;
   b.w = k0 + 3 < cnt ? b.w : b.x;
   return b;
+}
+
+// Near-entry ids of the three-body passes: per atom one 16-B word with the
+// list words of its first four near entries (kNidEnd past the last), written
+// by the pass that builds the near list.  A leg pass finds its own entry in a
+// neighbour's near list from that word (the entry of l at c is l with the
+// mirrored stencil code) and only reads further near entries of c when c has
+// more than four.
+constexpr uint32_t kNidEnd = 0xFFFFFFFFu;
+
+// Add the list word e of near entry k to the first-four word.
+__device__ inline void nid_add(uint4& w, int k, uint32_t e) {
+  w.x = k == 0 ? e : w.x;
+  w.y = k == 1 ? e : w.y;
+  w.z = k == 2 ? e : w.z;
+  w.w = k == 3 ? e : w.w;
+}
+
+// The list word of atom l as a neighbour of c, given l's entry e for c.
+__device__ inline uint32_t mirror_entry(int l, uint32_t e) {
+  return (uint32_t)l | ((26u - (e >> kImgShift)) << kImgShift);
+}
+
+// Near slot of list word `want` in c's near list (near: slot-major rows of
+// n, count: near-list lengths), or -1.
+__device__ inline int find_near_slot(const uint4* nid, const float4* near, const int* count,
+                                     int c, int n, uint32_t want) {
+  const uint4 w = nid[c];
+  if (w.x == want) return 0;
+  if (w.y == want) return 1;
+  if (w.z == want) return 2;
+  if (w.w == want) return 3;
+  if (w.w == kNidEnd) return -1;  // fewer than four entries
+  const int nc = count[c];
+  for (int s = 4; s < nc; s++)
+    if (__float_as_uint(near[(size_t)s * n + c].w) == want) return s;
+  return -1;
 }
 
 // The w word of an FP32 snapshot holds the atom type (2 bits) and the cell

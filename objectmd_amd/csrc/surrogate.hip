@@ -48,6 +48,7 @@ struct SgArgs {
   const int* nnbr;
   float4* near;
   int* near_n;
+  uint4* nid;     // list words of the first four near entries per atom
   float4* gsum;
   float* G;      // n x 32
   float* dEdG;   // n x 32
@@ -125,6 +126,7 @@ __global__ void __launch_bounds__(kBlock) k_sg_desc(SgArgs a) {
 #pragma unroll
     for (int g = 0; g < ND; g++) G[g] = 0.f;
     int nn = 0;
+    uint4 ids = make_uint4(kNidEnd, kNidEnd, kNidEnd, kNidEnd);
     const int cnt = a.nnbr[l];
     for (int k0 = 0; k0 < cnt; k0 += 4) {
       // a block of four list words, then their four snapshots, in flight together
@@ -148,6 +150,7 @@ __global__ void __launch_bounds__(kBlock) k_sg_desc(SgArgs a) {
         }
         if (nn < kNear) a.near[(size_t)nn * n + l] = make_float4(dx, dy, dz, __uint_as_float(e));
         else atomicOr(a.err, ERR_NEAR);
+        nid_add(ids, nn, e);
         nn++;
       }
     }
@@ -177,6 +180,7 @@ __global__ void __launch_bounds__(kBlock) k_sg_desc(SgArgs a) {
       }
     }
     a.near_n[l] = nn;
+    a.nid[l] = ids;
     float4* gout = reinterpret_cast<float4*>(a.G + (size_t)l * ND);
 #pragma unroll
     for (int g = 0; g < ND; g += 4) gout[g / 4] = make_float4(G[g], G[g + 1], G[g + 2], G[g + 3]);
@@ -409,20 +413,14 @@ __global__ void __launch_bounds__(kBlock) k_sg_legs(SgArgs a) {
   const int nn = a.near_n[l];
   float fx = a.f[l], fy = a.f[(size_t)n + l], fz = a.f[2 * (size_t)n + l];
   for (int u = 0; u < nn; u++) {
-    const float4 du = a.near[(size_t)u * n + l];
-    const int c = (int)(__float_as_uint(du.w) & kSlotMask);
-    const int nc = a.near_n[c];
-    for (int s = 0; s < nc; s++) {
-      const float4 dk = a.near[(size_t)s * n + c];  // x_k - x_c
-      if ((int)(__float_as_uint(dk.w) & kSlotMask) != l) continue;
-      const float ex = dk.x + du.x, ey = dk.y + du.y, ez = dk.z + du.z;
-      if (ex * ex + ey * ey + ez * ez >= 1e-4f) continue;  // another image of l
-      const float4 g = a.gsum[(size_t)s * n + c];
-      fx -= g.x;
-      fy -= g.y;
-      fz -= g.z;
-      break;
-    }
+    const uint32_t e = __float_as_uint(a.near[(size_t)u * n + l].w);
+    const int c = (int)(e & kSlotMask);
+    const int sc = find_near_slot(a.nid, a.near, a.near_n, c, n, mirror_entry(l, e));
+    if (sc < 0) continue;
+    const float4 g = a.gsum[(size_t)sc * n + c];
+    fx -= g.x;
+    fy -= g.y;
+    fz -= g.z;
   }
   a.f[l] = fx;
   a.f[(size_t)n + l] = fy;
@@ -512,6 +510,7 @@ void surrogate_forces(Ctx& c, bool energy) {
   a.nnbr = c.nnbr;
   a.near = c.near;
   a.near_n = c.near_n;
+  a.nid = c.nid;
   a.gsum = c.gsum;
   a.G = m.G;
   a.dEdG = m.dEdG;

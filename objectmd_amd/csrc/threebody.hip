@@ -51,8 +51,8 @@ struct TBArgs {
   float4* near;   // near list, kMaxNear slot rows of n (x_j - x_l, entry bits)
   float4* gsum;   // per near entry, force on that leg from the terms of its centre
   int* near_n;    // near-list length per atom
+  uint4* nid;     // list words of the first four near entries per atom
   const int* only_if;  // Tersoff passes run only when *only_if != 0 (if set)
-  int leg_min;  // centres with fewer near neighbours store no leg forces (SW 2, Tersoff 1)
   int* err;
   int n;
   float hx, hy, hz;
@@ -171,6 +171,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_pair(TBArgs a) {
   Kahan fx, fy, fz;
   float pot = 0.f, vir = 0.f;
   int nn = 0;
+  uint4 ids = make_uint4(kNidEnd, kNidEnd, kNidEnd, kNidEnd);
   const int cnt = a.nnbr[l];
   for (int k0 = 0; k0 < cnt; k0 += 4) {
     // a block of four list words, then their four snapshots, in flight together
@@ -196,6 +197,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_pair(TBArgs a) {
         if (r2 < a.rc2) {
           if (nn < kMaxNear) nl[nn] = make_float4(d.x, d.y, d.z, __uint_as_float(e));
           else atomicOr(a.err, ERR_NEAR);
+          nid_add(ids, nn, e);
           nn++;
         }
       } else {
@@ -230,6 +232,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_pair(TBArgs a) {
     if (ENERGY) pot += eh;
   }
   a.near_n[l] = nn;
+  a.nid[l] = ids;
   a.f[l] = fx.s;
   a.f[(size_t)n + l] = fy.s;
   a.f[2 * (size_t)n + l] = fz.s;
@@ -277,8 +280,11 @@ __global__ void __launch_bounds__(kBlock) k_sw_centre(TBArgs a) {
   if (i >= a.n) return;
   const int n = a.n;
   const int nn = a.near_n[i];
-  if (nn < 2) return;
   const Col ni{a.near + i, (size_t)n}, gi{a.gsum + i, (size_t)n};
+  if (nn < 2) {  // no triplet: a lone leg gets nothing
+    if (nn == 1) gi[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
   Kahan fx, fy, fz;
   fx.s = a.f[i];
   fy.s = a.f[(size_t)n + i];
@@ -311,7 +317,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_centre(TBArgs a) {
 
 // Pass 3 (SW and Tersoff): atom l collects, from each near neighbour c, the
 // force and energy share of the terms centred at c with l as a leg
-// (gsum[c][slot of l]).
+// (gsum[c][slot of l]; the slot from c's near ids).
 template <bool ENERGY>
 __global__ void __launch_bounds__(kBlock) k_tb_legs(TBArgs a) {
   if (a.only_if && !*a.only_if) return;
@@ -327,23 +333,15 @@ __global__ void __launch_bounds__(kBlock) k_tb_legs(TBArgs a) {
   fz.s = a.f[2 * (size_t)n + l];
   float pot = 0.f;
   for (int u = 0; u < nn; u++) {
-    const float4 du = nl[u];
-    const int c = (int)(__float_as_uint(du.w) & kSlotMask);
-    const int nc = a.near_n[c];
-    if (nc < a.leg_min) continue;
-    const Col ncl{a.near + c, (size_t)n};
-    for (int s = 0; s < nc; s++) {
-      const float4 dk = ncl[s];  // x_k - x_c
-      if ((int)(__float_as_uint(dk.w) & kSlotMask) != l) continue;
-      const float ex = dk.x + du.x, ey = dk.y + du.y, ez = dk.z + du.z;
-      if (ex * ex + ey * ey + ez * ez >= 1e-4f) continue;  // another image of l
-      const float4 g = a.gsum[(size_t)s * n + c];
-      fx.add(g.x);
-      fy.add(g.y);
-      fz.add(g.z);
-      if (ENERGY) pot += g.w;
-      break;
-    }
+    const uint32_t e = __float_as_uint(nl[u].w);
+    const int c = (int)(e & kSlotMask);
+    const int sc = find_near_slot(a.nid, a.near, a.near_n, c, n, mirror_entry(l, e));
+    if (sc < 0) continue;
+    const float4 g = a.gsum[(size_t)sc * n + c];
+    fx.add(g.x);
+    fy.add(g.y);
+    fz.add(g.z);
+    if (ENERGY) pot += g.w;
   }
   a.f[l] = fx.s;
   a.f[(size_t)n + l] = fy.s;
@@ -453,6 +451,7 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_bond(TBArgs a) {
   const int cnt = a.nnbr[i];
   const Col ni{a.near + i, (size_t)n};
   int nn = 0;
+  uint4 ids = make_uint4(kNidEnd, kNidEnd, kNidEnd, kNidEnd);
   for (int k0 = 0; k0 < cnt; k0 += 4) {
     const uint4 eb = list_block(a.nbr, i, k0, n, cnt);
     const uint32_t ev[4] = {eb.x, eb.y, eb.z, eb.w};
@@ -472,12 +471,14 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_bond(TBArgs a) {
       if (r2 < a.rc2) {
         if (nn < kMaxNear) ni[nn] = make_float4(d.x, d.y, d.z, __uint_as_float(e));
         else atomicOr(a.err, ERR_NEAR);
+        nid_add(ids, nn, e);
         nn++;
       }
     }
   }
   if (nn > kMaxNear) nn = kMaxNear;
   a.near_n[i] = nn;
+  a.nid[i] = ids;
   const TCon tk = tcon(p);
   float pot = 0.f;
   for (int u = 0; u < nn; u++) {
@@ -604,11 +605,14 @@ void near_arrays(Ctx& c, int k) {
   }
   if (c.near_n) (void)hipFree(c.near_n);
   c.near_n = nullptr;
+  if (c.nid) (void)hipFree(c.nid);
+  c.nid = nullptr;
   c.cap_near = 0;
   const size_t bytes = sizeof(float4) * k * (size_t)c.n;
   OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.near), bytes));
   OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.gsum), bytes));
   OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.near_n), sizeof(int) * (size_t)c.n));
+  OMDG_HIP(hipMalloc(reinterpret_cast<void**>(&c.nid), sizeof(uint4) * (size_t)c.n));
   c.cap_near = (size_t)c.n * k;
 }
 
@@ -666,7 +670,7 @@ void sw_forces(Ctx& c, bool energy) {
   a.near = c.near;
   a.gsum = c.gsum;
   a.near_n = c.near_n;
-  a.leg_min = 2;  // the centre pass stores nothing for centres without a triplet
+  a.nid = c.nid;
   const int nb = grid_blocks(c.n, kBlock);
   {
     TimerScope ts(c, TG_PASS1);
@@ -707,7 +711,7 @@ void tersoff_forces_if(Ctx& c, bool energy, const int* only_if) {
   a.near = c.near;
   a.gsum = c.gsum;
   a.near_n = c.near_n;
-  a.leg_min = 1;  // a lone bond still has its pair term
+  a.nid = c.nid;
   const int nb = grid_blocks(c.n, kBlock);
   {
     TimerScope ts(c, TG_PASS1);
