@@ -421,6 +421,19 @@ __device__ inline float ters_t(const TCon& k, const TLeg& j, const TLeg& q, floa
     g3[0] = a * j.ux + b * q.ux;
     g3[1] = a * j.uy + b * q.uy;
     g3[2] = a * j.uz + b * q.uz;
+  } else if (GRAD == 3) {
+    // both: g3[0..2] = dT/dd_j, g3[3..5] = dT/dd_k
+    const float dgc = q.fc * ex * (-2.f * k.c2 * hc * iden * iden);
+    const float aj = dgc * j.ir, ak = dgc * q.ir;
+    const float dx3 = k.l3m * 3.f * dl * dl;
+    const float bj = q.fc * g * ex * dx3 - aj * cs;
+    const float bk = g * ex * (q.dfc - q.fc * dx3) - ak * cs;
+    g3[0] = aj * q.ux + bj * j.ux;
+    g3[1] = aj * q.uy + bj * j.uy;
+    g3[2] = aj * q.uz + bj * j.uz;
+    g3[3] = ak * j.ux + bk * q.ux;
+    g3[4] = ak * j.uy + bk * q.uy;
+    g3[5] = ak * j.uz + bk * q.uz;
   }
   return q.fc * g * ex;
 }
@@ -431,6 +444,35 @@ __device__ inline float ters_pair(const TersoffParams& p, const TLeg& L, float b
   const float fA = -p.B * expf(-p.lambda2 * L.r);
   const float dV = L.dfc * (fR + b * fA) + L.fc * (-p.lambda1 * fR - p.lambda2 * b * fA);
   return 0.5f * dV * L.ir;
+}
+
+// Bond order of bond (i,j) from its zeta: b_ij, P_ij = 1/2 fc fA db/dzeta and
+// the bond energy 1/2 V_ij = 1/2 fc (fR + b fA).
+__device__ inline void ters_bond_order(const TersoffParams& p, const TLeg& L, float zeta,
+                                       float& b, float& P, float& ehalf) {
+  const float fA = -p.B * expf(-p.lambda2 * L.r);
+  float db = 0.f;
+  b = 1.f;
+  if (zeta > 0.f) {
+    // x = (beta zeta)^n overflows FP32 for close pairs: work with
+    // lx = log x.  b = (1+x)^(-1/2n) = exp(-log1p(x) / 2n) and
+    // db/dzeta = -1/2 (1+x)^(-1/2n - 1) beta^n zeta^(n-1)
+    //          = -b x / (2 zeta (1+x)).
+    const float lx = p.n * logf(p.beta * zeta);
+    float l1p, t;
+    if (lx > 30.f) {
+      l1p = lx;  // log1p(x) = lx + log1p(1/x), 1/x < 1e-13
+      t = 1.f;
+    } else {
+      const float x = expf(lx);
+      l1p = log1pf(x);
+      t = x / (1.f + x);
+    }
+    b = expf(-0.5f / p.n * l1p);
+    db = -0.5f * b * t / zeta;
+  }
+  P = 0.5f * L.fc * fA * db;
+  ehalf = 0.5f * L.fc * (p.A * expf(-p.lambda1 * L.r) + b * fA);
 }
 
 // Pass 1: the list scan writes the near list {x_j - x_i, entry} of the
@@ -481,39 +523,42 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_bond(TBArgs a) {
   a.nid[i] = ids;
   const TCon tk = tcon(p);
   float pot = 0.f;
-  for (int u = 0; u < nn; u++) {
-    const TLeg Lu = tleg(p, ni[u]);
-    const float rij = Lu.r, fc = Lu.fc;
-    float zeta = 0.f;
-    for (int w = 0; w < nn; w++) {
-      if (w == u) continue;
-      zeta += ters_t<0>(tk, Lu, tleg(p, ni[w]), nullptr);
+  if (nn <= 4) {
+    // up to four bonds (a crystal's first shell): legs in registers
+    TLeg L[4];
+    float z[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      L[u] = tleg(p, ni[u < nn ? u : 0]);  // unconditional load
+      z[u] = 0.f;
     }
-    const float fA = -p.B * expf(-p.lambda2 * rij);
-    float b = 1.f, db = 0.f;
-    if (zeta > 0.f) {
-      // x = (beta zeta)^n overflows FP32 for close pairs: work with
-      // lx = log x.  b = (1+x)^(-1/2n) = exp(-log1p(x) / 2n) and
-      // db/dzeta = -1/2 (1+x)^(-1/2n - 1) beta^n zeta^(n-1)
-      //          = -b x / (2 zeta (1+x)).
-      const float lx = p.n * logf(p.beta * zeta);
-      float l1p, t;
-      if (lx > 30.f) {
-        l1p = lx;  // log1p(x) = lx + log1p(1/x), 1/x < 1e-13
-        t = 1.f;
-      } else {
-        const float x = expf(lx);
-        l1p = log1pf(x);
-        t = x / (1.f + x);
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int w = 0; w < 4; w++)
+        if (w != u && u < nn && w < nn) z[u] += ters_t<0>(tk, L[u], L[w], nullptr);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (u >= nn) continue;
+      float bo, P, eh;
+      ters_bond_order(p, L[u], z[u], bo, P, eh);
+      a.bond_b[(size_t)u * n + i] = bo;
+      a.bond_p[(size_t)u * n + i] = P;
+      if (ENERGY) pot += eh;
+    }
+  } else {
+    for (int u = 0; u < nn; u++) {
+      const TLeg Lu = tleg(p, ni[u]);
+      float zeta = 0.f;
+      for (int w = 0; w < nn; w++) {
+        if (w == u) continue;
+        zeta += ters_t<0>(tk, Lu, tleg(p, ni[w]), nullptr);
       }
-      b = expf(-0.5f / p.n * l1p);
-      db = -0.5f * b * t / zeta;
-    }
-    a.bond_b[(size_t)u * n + i] = b;
-    a.bond_p[(size_t)u * n + i] = 0.5f * fc * fA * db;
-    if (ENERGY) {
-      const float fR = p.A * expf(-p.lambda1 * rij);
-      pot += 0.5f * fc * (fR + b * fA);
+      float bo, P, eh;
+      ters_bond_order(p, Lu, zeta, bo, P, eh);
+      a.bond_b[(size_t)u * n + i] = bo;
+      a.bond_p[(size_t)u * n + i] = P;
+      if (ENERGY) pot += eh;
     }
   }
   if (ENERGY) {
@@ -538,6 +583,52 @@ __global__ void __launch_bounds__(kBlock) k_tersoff_centre(TBArgs a) {
   const Col ni{a.near + i, (size_t)n}, gi{a.gsum + i, (size_t)n};
   const TCon tk = tcon(p);
   Kahan fx, fy, fz;
+  if (nn <= 4) {
+    // up to four legs (a crystal's first shell): legs in registers, each
+    // ordered pair (u, w) evaluated once for both its gradients (bond u with
+    // u as the j-atom, w as the k-atom)
+    float4 d[4];
+    TLeg L[4];
+    float P[4], S[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int uu = u < nn ? u : 0;  // unconditional loads
+      d[u] = ni[uu];
+      L[u] = tleg(p, d[u]);
+      P[u] = u < nn ? a.bond_p[(size_t)uu * n + i] : 0.f;
+      const float pr = u < nn ? ters_pair(p, L[u], a.bond_b[(size_t)uu * n + i]) : 0.f;
+      S[u][0] = pr * d[u].x;
+      S[u][1] = pr * d[u].y;
+      S[u][2] = pr * d[u].z;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+#pragma unroll
+      for (int w = 0; w < 4; w++) {
+        if (w == u || u >= nn || w >= nn || P[u] == 0.f) continue;
+        float g6[6];
+        ters_t<3>(tk, L[u], L[w], g6);
+        S[u][0] += P[u] * g6[0];
+        S[u][1] += P[u] * g6[1];
+        S[u][2] += P[u] * g6[2];
+        S[w][0] += P[u] * g6[3];
+        S[w][1] += P[u] * g6[4];
+        S[w][2] += P[u] * g6[5];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (u >= nn) continue;
+      gi[u] = make_float4(-S[u][0], -S[u][1], -S[u][2], 0.f);
+      fx.add(S[u][0]);
+      fy.add(S[u][1]);
+      fz.add(S[u][2]);
+    }
+    a.f[i] = fx.s;
+    a.f[(size_t)n + i] = fy.s;
+    a.f[2 * (size_t)n + i] = fz.s;
+    return;
+  }
   for (int u = 0; u < nn; u++) {
     const float4 du = ni[u];
     const TLeg Lu = tleg(p, du);
