@@ -352,6 +352,73 @@ __global__ void __launch_bounds__(kBlock) k_sg_centre(SgArgs a) {
     dg[g + 3] = t.w;
   }
   float fx = 0.f, fy = 0.f, fz = 0.f;
+  if (nn <= 4) {
+    // up to four legs (the first shell of a crystal): legs in registers,
+    // each unordered leg pair evaluated once for both of its legs
+    float4 d[4];
+    float r[4], ir[4], fcu[4], dfcu[4], G[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      d[u] = a.near[(size_t)(u < nn ? u : 0) * n + i];  // unconditional load
+      r[u] = sqrtf(d[u].x * d[u].x + d[u].y * d[u].y + d[u].z * d[u].z);
+      ir[u] = __builtin_amdgcn_rcpf(r[u]);
+      cutoff(a, r[u], fcu[u], dfcu[u]);
+      // radial: sum_k dE/dG_k phi_k'(r) / r
+      float coef = 0.f;
+#pragma unroll
+      for (int k = 0; k < NR; k++) {
+        const float t = r[u] - (a.mu0 + a.dmu * (float)k);
+        coef += dg[k] * __expf(-a.eta_r * t * t) * (-2.f * a.eta_r * t * fcu[u] + dfcu[u]);
+      }
+      coef *= ir[u];
+      G[u][0] = coef * d[u].x;
+      G[u][1] = coef * d[u].y;
+      G[u][2] = coef * d[u].z;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+#pragma unroll
+      for (int w = u + 1; w < 4; w++) {
+        if (w >= nn) continue;
+        const float cs = (d[u].x * d[w].x + d[u].y * d[w].y + d[u].z * d[w].z) * ir[u] * ir[w];
+        const float ea = __expf(-a.eta_a * (r[u] * r[u] + r[w] * r[w]));
+        const float rad = ea * fcu[u] * fcu[w];
+        float sa = 0.f, sda = 0.f;  // sum_m dE/dG_m A_m and dA_m/dcos
+#pragma unroll
+        for (int m = 0; m < NA; m++) {
+          float dA;
+          const float A = ang(a, m, cs, dA);
+          sa += dg[NR + m] * A;
+          sda += dg[NR + m] * dA;
+        }
+        const float sc = sda * rad;  // coefficient of d cos
+        const float ciw = sc * ir[u] * ir[w];
+        // leg u: d cos/d x_u = d_w / (r_u r_w) - cos d_u / r_u^2, d rad/d x_u
+        const float cu = sa * (-2.f * a.eta_a * rad + ea * dfcu[u] * fcu[w] * ir[u]) -
+                         sc * cs * ir[u] * ir[u];
+        const float cw = sa * (-2.f * a.eta_a * rad + ea * fcu[u] * dfcu[w] * ir[w]) -
+                         sc * cs * ir[w] * ir[w];
+        G[u][0] += ciw * d[w].x + cu * d[u].x;
+        G[u][1] += ciw * d[w].y + cu * d[u].y;
+        G[u][2] += ciw * d[w].z + cu * d[u].z;
+        G[w][0] += ciw * d[u].x + cw * d[w].x;
+        G[w][1] += ciw * d[u].y + cw * d[w].y;
+        G[w][2] += ciw * d[u].z + cw * d[w].z;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (u >= nn) continue;
+      a.gsum[(size_t)u * n + i] = make_float4(G[u][0], G[u][1], G[u][2], 0.f);
+      fx += G[u][0];
+      fy += G[u][1];
+      fz += G[u][2];
+    }
+    a.f[i] = fx;
+    a.f[(size_t)n + i] = fy;
+    a.f[2 * (size_t)n + i] = fz;
+    return;
+  }
   for (int u = 0; u < nn; u++) {
     const float4 du = a.near[(size_t)u * n + i];
     const float ru = sqrtf(du.x * du.x + du.y * du.y + du.z * du.z);
