@@ -17,13 +17,18 @@
 //
 // LANES lanes serve one atom: lane j walks list entries j, j+LANES, ... with
 // a Kahan-compensated FP32 sum, and the compensated partials are added in
-// FP64 across the lanes.  Each lane requests the list words, snapshots and
-// table records of CHUNK entries before using any of them, so a wavefront
-// keeps many independent loads in flight; the segment lookup is branch-free
-// and range errors reach the status word once per lane.  When PAIR and
-// EDENS share one grid their coefficients are interleaved per interval
-// (32 B, one cache line per pair lookup).  (LANES, CHUNK) is picked per
-// context by timing the candidates on the first evaluation.
+// FP64 across the lanes.  Each lane first requests all its list words of a
+// stretch (8-16), then the snapshots and table records of CHUNK entries
+// before using any of them, so a wavefront keeps many independent loads in
+// flight.  Every gather is unconditional (an invalid entry re-reads a valid
+// slot and is masked afterwards): a conditional load would compile to a
+// branch and a separate load per 32-bit component.  The force pass reads the
+// snapshot and F'(rho) of a neighbour in one 16-B gather ({q.xyz, F'} written
+// by the density pass).  The segment lookup is branch-free and the table
+// range is checked once per lane from the extremes of its lookups.  When
+// PAIR and EDENS share one grid their coefficients are interleaved per
+// interval (32 B, one cache line per pair lookup).  (LANES, CHUNK) is picked
+// per context by timing the candidates on the first evaluation.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
