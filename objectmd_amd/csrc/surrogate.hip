@@ -243,16 +243,28 @@ __global__ void __launch_bounds__(kBlock) k_sg_mlp(SgArgs a) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
   const int ngroups = (a.n + 15) / 16;
+  // descriptors of the lane's atom in group gi (features 16 kb + 4 grp ..
+  // + 3); atoms past the end re-read the last atom (unconditional loads)
+  auto load_g = [&](int gi, float4* gv) {
+    const int atom = min(gi * 16 + col, a.n - 1);
+#pragma unroll
+    for (int kb = 0; kb < 2; kb++)
+      gv[kb] = *reinterpret_cast<const float4*>(a.G + (size_t)atom * ND + kb * 16 + 4 * grp);
+  };
+  float4 gnext[2];
+  if (wave < ngroups) load_g(wave, gnext);
   for (int gi = wave; gi < ngroups; gi += nwaves) {
     const int atom = gi * 16 + col;
     const bool live = atom < a.n;
+    float4 gcur[2] = {gnext[0], gnext[1]};
+    // the next group's descriptors are in flight while this group computes
+    load_g(gi + nwaves < ngroups ? gi + nwaves : gi, gnext);
     // B operand of layer 1: z features 16 kb + 4 grp + item of this atom
     bf16x4 bz[2];
 #pragma unroll
     for (int kb = 0; kb < 2; kb++) {
       const int g0 = kb * 16 + 4 * grp;
-      float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (live) gv = *reinterpret_cast<const float4*>(a.G + (size_t)atom * ND + g0);
+      const float4 gv = gcur[kb];
       bz[kb] = pack((gv.x - v[kMean + g0]) * v[kIstd + g0],
                     (gv.y - v[kMean + g0 + 1]) * v[kIstd + g0 + 1],
                     (gv.z - v[kMean + g0 + 2]) * v[kIstd + g0 + 2],
@@ -621,7 +633,10 @@ void surrogate_forces(Ctx& c, bool energy) {
   {
     TimerScope ts(c, TG_MLP);
     const int groups = (n + 15) / 16;
-    k_sg_mlp<<<std::min(2048, std::max(1, (groups + 3) / 4)), kBlock, 0, c.stream>>>(a);
+    // grid-stride over groups of 16 atoms; the weight fragments stay in
+    // registers (two waves per SIMD), so 512 blocks of four waves fill the
+    // 256 CUs
+    k_sg_mlp<<<std::min(512, std::max(1, (groups + 3) / 4)), kBlock, 0, c.stream>>>(a);
   }
   {
     TimerScope ts(c, TG_FORCE);
