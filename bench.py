@@ -259,7 +259,7 @@ def bench_tersoff(args):
     x = wrap(pos + rng.normal(0.0, 0.05, pos.shape), lo, L, 3)
     v = lattice.maxwell_boltzmann(len(x), m, 1000.0, seed + 1)
     n = len(x)
-    skin = 0.5 if args.skin is None else args.skin
+    skin = 0.7 if args.skin is None else args.skin
     steps = 1000 if args.steps is None else args.steps
     warmup = 20 if args.warmup is None else args.warmup
     tp = params.tersoff_params(os.path.join(REF, "tersoff.silicon"), 2.7, 0.15)
@@ -399,7 +399,7 @@ def bench_surrogate(args):
     x = wrap(pos + rng.normal(0.0, 0.05, pos.shape), lo, L, 3)
     v = lattice.maxwell_boltzmann(len(x), m, 1000.0, seed + 1)
     n = len(x)
-    skin = 0.5 if args.skin is None else args.skin
+    skin = 0.7 if args.skin is None else args.skin
     steps = 10000 if args.steps is None else args.steps
     warmup = 20 if args.warmup is None else args.warmup
     tp = params.tersoff_params(os.path.join(REF, "tersoff.silicon"), 2.7, 0.15)
