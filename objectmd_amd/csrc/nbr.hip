@@ -431,26 +431,60 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a) {
           return ddx * ddx + ddy * ddy + ddz * ddz <= a.rl2;
         };
         if (L == 1) {
-          // four candidate snapshots in flight; entries gather in a block of
-          // four and leave as one 16-B store
+          // Four candidates per round, tested into bit masks with few
+          // branches: i's snapshot moved into the frame of this cell, the
+          // FP64 image shifts of the cell (x + L or x + (-L) as in shifted)
+          const float px = qi.x - o.x, py = qi.y - o.y, pz = qi.z - o.z;
+          const double shx = sx == 0 ? 0.0 : (sx < 0 ? -a.g.L[0] : a.g.L[0]);
+          const double shy = sy == 0 ? 0.0 : (sy < 0 ? -a.g.L[1] : a.g.L[1]);
+          const double shz = sz == 0 ? 0.0 : (sz < 0 ? -a.g.L[2] : a.g.L[2]);
           for (int t0 = a.start[cc]; t0 < e; t0 += 4) {
             float4 qt[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) qt[u] = a.q[t0 + u < e ? t0 + u : e - 1];
+            unsigned in = 0u, band = 0u;
 #pragma unroll
             for (int u = 0; u < 4; u++) {
               const int t = t0 + u;
-              if (t < e && member(t, qt[u])) {
-                const uint32_t ent = (uint32_t)t | (img << kImgShift);
-                const int slot = cnt & 3;
-                blk.x = slot == 0 ? ent : blk.x;
-                blk.y = slot == 1 ? ent : blk.y;
-                blk.z = slot == 2 ? ent : blk.z;
-                blk.w = slot == 3 ? ent : blk.w;
-                cnt++;
-                if (slot == 3 && cnt <= a.maxnbr)
-                  *reinterpret_cast<uint4*>(a.nbr + nidx(s, cnt - 4, n)) = blk;
+              const float fx = px - qt[u].x, fy = py - qt[u].y, fz = pz - qt[u].z;
+              const float r2f = fx * fx + fy * fy + fz * fz;
+              const bool ok = t < e && (t != s || !noshift);
+              in |= (ok && r2f < a.rl2_lo) ? (1u << u) : 0u;
+              band |= (ok && r2f >= a.rl2_lo && r2f <= a.rl2_hi) ? (1u << u) : 0u;
+            }
+            if (a.has_flags && (in | band)) {
+#pragma unroll
+              for (int u = 0; u < 4; u++)
+                if (((in | band) >> u) & 1u)
+                  if (!(a.flag[t0 + u] & OMDG_FLAG_ACTIVE)) {
+                    in &= ~(1u << u);
+                    band &= ~(1u << u);
+                  }
+            }
+            if (band) {  // inside the band the reference's FP64 test decides
+#pragma unroll
+              for (int u = 0; u < 4; u++) {
+                if (!((band >> u) & 1u)) continue;
+                const int t = t0 + u;
+                const double ddx = xi - (a.x[t] + shx);
+                const double ddy = yi - (a.x[(size_t)n + t] + shy);
+                const double ddz = zi - (a.x[2 * (size_t)n + t] + shz);
+                if (ddx * ddx + ddy * ddy + ddz * ddz <= a.rl2) in |= 1u << u;
               }
+            }
+            // entries gather in a block of four and leave as one 16-B store
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              if (!((in >> u) & 1u)) continue;
+              const uint32_t ent = (uint32_t)(t0 + u) | (img << kImgShift);
+              const int slot = cnt & 3;
+              blk.x = slot == 0 ? ent : blk.x;
+              blk.y = slot == 1 ? ent : blk.y;
+              blk.z = slot == 2 ? ent : blk.z;
+              blk.w = slot == 3 ? ent : blk.w;
+              cnt++;
+              if (slot == 3 && cnt <= a.maxnbr)
+                *reinterpret_cast<uint4*>(a.nbr + nidx(s, cnt - 4, n)) = blk;
             }
           }
         } else {
