@@ -16,7 +16,8 @@
 // rounding is that of |q| ~ one cell, not of the absolute coordinate.
 //
 // LANES lanes serve one atom: lane j walks list entries j, j+LANES, ... with
-// a Kahan-compensated FP32 sum, and the compensated partials are added in
+// an FP32 partial sum (Kahan-compensated for the forces, plain for the
+// densities, whose terms are all positive), and the partials are added in
 // FP64 across the lanes.  Each lane first requests all its list words of a
 // stretch (8-16), then the snapshots and table records of CHUNK entries
 // before using any of them, so a wavefront keeps many independent loads in
@@ -43,7 +44,6 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr int kCentre = 13;  // image code of the zero offset
 
 struct EamArgs {
   const float4* q;
@@ -63,13 +63,18 @@ struct EamArgs {
   const float4* pack;  // {phi a,b,c,d}, {rho a,b,c,d} per interval, or null
 };
 
-// Compensated partial -> FP64, summed over the LANES lanes of one atom.
+// Per-lane partial in FP64, summed over the LANES lanes of one atom.
 template <int LANES>
-__device__ inline double lanes_sum(const Kahan& k) {
-  double v = (double)k.s - (double)k.c;
+__device__ inline double lanes_sum(double v) {
 #pragma unroll
   for (int o = 1; o < LANES; o <<= 1) v += __shfl_xor(v, o, 64);
   return v;
+}
+
+// Compensated partial -> FP64, summed over the LANES lanes of one atom.
+template <int LANES>
+__device__ inline double lanes_sum(const Kahan& k) {
+  return lanes_sum<LANES>((double)k.s - (double)k.c);
 }
 
 // list words one lane requests before using any: a stretch of LANES * kPre
@@ -83,8 +88,9 @@ constexpr int pre_words() {
 template <bool ENERGY, int LANES, int CHUNK>
 __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
   constexpr int kPre = pre_words<LANES, CHUNK>();
-  __shared__ float4 offs[27];
+  __shared__ float4 offs[32];  // codes 27..31 (the empty entry's 31) read zero
   load_offsets(offs, a.hx, a.hy, a.hz);
+  if (threadIdx.x >= 27 && threadIdx.x < 32) offs[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
   __syncthreads();
   const int g = xcd_block() * blockDim.x + threadIdx.x;
   const int s = g / LANES, j = g % LANES;
@@ -95,7 +101,7 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
   const int nn = live ? a.nnbr[s] : 0;
   int err = 0;
   QRange qe;
-  Kahan rho;
+  float rho = 0.f;  // positive terms: a plain FP32 partial per lane
   for (int b0 = j; b0 < nn; b0 += LANES * kPre) {
     // all list words of this stretch in flight at once
     uint32_t ew[kPre];
@@ -120,7 +126,7 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
       int seg[CHUNK];
 #pragma unroll
       for (int u = 0; u < CHUNK; u++) {
-        const float4 o = offs[e[u] != kNone ? (e[u] >> kImgShift) : kCentre];
+        const float4 o = offs[e[u] >> kImgShift];
         const float ex = qi.x - qj[u].x - o.x;
         const float ey = qi.y - qj[u].y - o.y;
         const float ez = qi.z - qj[u].z - o.z;
@@ -136,13 +142,13 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
       for (int u = 0; u < CHUNK; u++) {
         float v = c[u].w + dx[u] * (c[u].z + dx[u] * (c[u].y + dx[u] * c[u].x));
         if (a.edens.rmult) v /= r[u];
-        rho.add(in[u] ? v : 0.f);
+        rho += in[u] ? v : 0.f;
       }
     }
   }
   err |= range_flags(a.edens, qe);
   if (err) atomicOr(a.err, err);
-  const double rs = lanes_sum<LANES>(rho);
+  const double rs = lanes_sum<LANES>((double)rho);
   if (live && j == 0) {
     a.rho[s] = (float)rs;
     // embedding function and derivative in FP64, once per atom
@@ -156,8 +162,9 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
 template <bool ENERGY, bool PACKED, int LANES, int CHUNK>
 __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
   constexpr int kPre = pre_words<LANES, CHUNK>();
-  __shared__ float4 offs[27];
+  __shared__ float4 offs[32];  // codes 27..31 (the empty entry's 31) read zero
   load_offsets(offs, a.hx, a.hy, a.hz);
+  if (threadIdx.x >= 27 && threadIdx.x < 32) offs[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
   __syncthreads();
   const int g = xcd_block() * blockDim.x + threadIdx.x;
   const int s = g / LANES, j = g % LANES;
@@ -194,7 +201,7 @@ __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
       int sp[CHUNK], se[CHUNK];
 #pragma unroll
       for (int u = 0; u < CHUNK; u++) {
-        const float4 o = offs[e[u] != kNone ? (e[u] >> kImgShift) : kCentre];
+        const float4 o = offs[e[u] >> kImgShift];
         d[u][0] = qi.x - qj[u].x - o.x;
         d[u][1] = qi.y - qj[u].y - o.y;
         d[u][2] = qi.z - qj[u].z - o.z;
