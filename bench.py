@@ -428,8 +428,11 @@ def bench_surrogate(args):
     flop_atom = 2 * (32 * 64 + 64 * 64 + 64) + 2 * (64 * 64 + 64 * 32)
     ms, launches = times["mlp"]
     tflops = flop_atom * n / (ms / max(launches, 1) / 1e3) / 1e12 if launches else 0.0
-    # force error against Tersoff on the final positions
+    # force error of the model against Tersoff on the final positions (the
+    # gauge limit lifted so that this evaluation is the model's, not the
+    # fallback's)
     xf = wrap(eng.download(om.X), lo, L, 3)
+    surrogate.configure(eng, model, desc, tp, threshold=1e30)
     eng.compute()
     fs = eng.download(om.F)
     ter = om.Engine()

@@ -300,56 +300,26 @@ __global__ void __launch_bounds__(kBlock) k_batch_build(BBArgs a) {
           return ddx * ddx + ddy * ddy + ddz * ddz <= a.rl2[pt];
         };
         if (L == 1) {
-          // Four packed candidates per round, tested into bit masks with few
-          // branches (i's snapshot moved into the frame of this cell; the
-          // FP64 image shifts of the cell as in shifted)
-          const float px = qi.x - ox, py = qi.y - oy, pz = qi.z - oz;
-          const double shx = sx == 0 ? 0.0 : (sx < 0 ? -g.L[0] : g.L[0]);
-          const double shy = sy == 0 ? 0.0 : (sy < 0 ? -g.L[1] : g.L[1]);
-          const double shz = sz == 0 ? 0.0 : (sz < 0 ? -g.L[2] : g.L[2]);
+          // four candidates in flight; entries gather in a block of four and
+          // leave as one 16-B store
           for (int u0 = a.start[cc]; u0 < e; u0 += 4) {
             float4 qt[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) qt[k] = a.qs[u0 + k < e ? u0 + k : e - 1];
-            unsigned in = 0u, band = 0u;
-            int tk[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-              const uint32_t w = __float_as_uint(qt[k].w);
-              tk[k] = (int)(w & kSlotMask);
-              const int pt = ti * kMaxTypes + (int)((w >> kCandTypeShift) & 3u);
-              const float fx = px - qt[k].x, fy = py - qt[k].y, fz = pz - qt[k].z;
-              const float r2f = fx * fx + fy * fy + fz * fz;
-              const bool ok = u0 + k < e && (tk[k] != s || !noshift) && (w & kCandActive);
-              in |= (ok && r2f < a.rl2_lo[pt]) ? (1u << k) : 0u;
-              band |= (ok && r2f >= a.rl2_lo[pt] && r2f <= a.rl2_hi[pt]) ? (1u << k) : 0u;
-            }
-            if (band) {  // inside the band the reference's FP64 test decides
-#pragma unroll
-              for (int k = 0; k < 4; k++) {
-                if (!((band >> k) & 1u)) continue;
-                const int t = tk[k];
-                const int pt =
-                    ti * kMaxTypes + (int)((__float_as_uint(qt[k].w) >> kCandTypeShift) & 3u);
-                const double ddx = xi - (a.x[t] + shx);
-                const double ddy = yi - (a.x[(size_t)n + t] + shy);
-                const double ddz = zi - (a.x[2 * (size_t)n + t] + shz);
-                if (ddx * ddx + ddy * ddy + ddz * ddz <= a.rl2[pt]) in |= 1u << k;
+              int t = 0;
+              if (u0 + k < e && member(qt[k], t)) {
+                const uint32_t ent = (uint32_t)t | (img << kImgShift);
+                const int slot = cnt & 3;
+                blk.x = slot == 0 ? ent : blk.x;
+                blk.y = slot == 1 ? ent : blk.y;
+                blk.z = slot == 2 ? ent : blk.z;
+                blk.w = slot == 3 ? ent : blk.w;
+                cnt++;
+                if (slot == 3 && cnt <= a.maxnbr)
+                  *reinterpret_cast<uint4*>(a.nbr + nidx(s, cnt - 4, n)) = blk;
               }
-            }
-            // entries gather in a block of four and leave as one 16-B store
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-              if (!((in >> k) & 1u)) continue;
-              const uint32_t ent = (uint32_t)tk[k] | (img << kImgShift);
-              const int slot = cnt & 3;
-              blk.x = slot == 0 ? ent : blk.x;
-              blk.y = slot == 1 ? ent : blk.y;
-              blk.z = slot == 2 ? ent : blk.z;
-              blk.w = slot == 3 ? ent : blk.w;
-              cnt++;
-              if (slot == 3 && cnt <= a.maxnbr)
-                *reinterpret_cast<uint4*>(a.nbr + nidx(s, cnt - 4, n)) = blk;
             }
           }
         } else {
