@@ -16,20 +16,24 @@
 // rounding is that of |q| ~ one cell, not of the absolute coordinate.
 //
 // LANES lanes serve one atom: lane j walks list entries j, j+LANES, ... with
-// an FP32 partial sum (Kahan-compensated for the forces, plain for the
-// densities, whose terms are all positive), and the partials are added in
-// FP64 across the lanes.  Each lane first requests all its list words of a
-// stretch (8-16), then the snapshots and table records of CHUNK entries
-// before using any of them, so a wavefront keeps many independent loads in
-// flight.  Every gather is unconditional (an invalid entry re-reads a valid
-// slot and is masked afterwards): a conditional load would compile to a
-// branch and a separate load per 32-bit component.  The force pass reads the
-// snapshot and F'(rho) of a neighbour in one 16-B gather ({q.xyz, F'} written
-// by the density pass).  The segment lookup is branch-free and the table
-// range is checked once per lane from the extremes of its lookups.  When
-// PAIR and EDENS share one grid their coefficients are interleaved per
-// interval (32 B, one cache line per pair lookup).  (LANES, CHUNK) is picked
-// per context by timing the candidates on the first evaluation.
+// a plain FP32 partial for the densities (all terms positive) and FP64
+// partials for the forces (CHUNK terms summed in FP32, then added in FP64),
+// and the partials are added in FP64 across the lanes.  Each lane first
+// requests all its list words of a stretch (8-16), then the snapshots and
+// table records of CHUNK entries before using any of them, so a wavefront
+// keeps many independent loads in flight.  Loads take a wavefront-uniform
+// base and a 32-bit lane offset (no 64-bit address arithmetic per lane).
+// Every snapshot gather is unconditional (entries past the count read the
+// atom itself and are masked afterwards); table records are loaded only for
+// pairs inside the cut, under the exec mask, since the address work of a
+// vector load scales with its active lanes (about a fifth of the list lies
+// in the skin).  The force pass reads the snapshot and F'(rho) of a
+// neighbour in one 16-B gather ({q.xyz, F'} written by the density pass).
+// The segment lookup is branch-free and the table range is checked once per
+// lane from the extremes of its lookups.  When PAIR and EDENS share one grid
+// their coefficients are interleaved per interval (32 B, one cache line per
+// pair lookup).  The shape (lanes per atom, entries in flight) is picked per
+// context by timing the candidates on the first evaluation.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -43,7 +47,6 @@ namespace omdg {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 struct EamArgs {
   const float4* q;
@@ -57,6 +60,7 @@ struct EamArgs {
   float* vir;
   int* err;
   int n;
+  int rows;  // list rows (capacity / 4)
   float cut_e2, cut_p2;
   float hx, hy, hz;
   TabView phi, edens, embed;
@@ -71,12 +75,6 @@ __device__ inline double lanes_sum(double v) {
   return v;
 }
 
-// Compensated partial -> FP64, summed over the LANES lanes of one atom.
-template <int LANES>
-__device__ inline double lanes_sum(const Kahan& k) {
-  return lanes_sum<LANES>((double)k.s - (double)k.c);
-}
-
 // list words one lane requests before using any: a stretch of LANES * kPre
 // entries costs one memory latency instead of one per CHUNK
 template <int LANES, int CHUNK>
@@ -85,73 +83,96 @@ constexpr int pre_words() {
   return w < CHUNK ? CHUNK : w;
 }
 
-template <bool ENERGY, int LANES, int CHUNK>
-__global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
-  constexpr int kPre = pre_words<LANES, CHUNK>();
-  __shared__ float4 offs[32];  // codes 27..31 (the empty entry's 31) read zero
+// Load at a 32-bit byte offset from a wavefront-uniform base: one
+// scalar-base + vector-offset access, no 64-bit address arithmetic per lane.
+template <class T>
+__device__ inline T ld_at(const T* base, uint32_t byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+// List words of LANES lanes per atom.  A stretch starts at entry kk0 (a
+// multiple of LANES, the same for the whole wavefront); lane j's entry
+// kk + j (kk = kk0 + t * LANES) lies in row (kk >> 2) + (j >> 2), word
+// (kk & 3) + (j & 3) of the blocked layout of nidx, since (kk & 3) + (j & 3)
+// stays below 4 for LANES = 2 and kk & 3 = 0 for LANES >= 4.  The row is a
+// uniform pointer, the lane part a 32-bit offset fixed for the kernel.  Rows
+// are clamped to the last whole group of LANES / 4 rows of the list (the
+// list's row count is a multiple of the group, checked at launch), so reads
+// past the end stay inside it; entries at or past the atom's count become
+// `pad`, the atom itself with stencil code 31 (zero offset).
+template <int LANES>
+struct ListLanes {
+  static constexpr int kGroup = LANES >= 4 ? LANES / 4 : 1;  // rows one stretch step spans
+  const uint32_t* nbr;
+  size_t row_words;  // 4 n
+  uint32_t lane_off;  // bytes: ((j >> 2) n + s) * 16 + (j & 3) * 4
+  int rmax;           // largest first row
+  int rem;            // entries of this lane: kk < rem
+  uint32_t pad;
+  __device__ ListLanes(const EamArgs& a, int sl, int j, int nn) {
+    nbr = a.nbr;
+    row_words = (size_t)a.n * 4;
+    lane_off = ((uint32_t)(j >> 2) * (uint32_t)a.n + (uint32_t)sl) * 16u + (uint32_t)(j & 3) * 4u;
+    rmax = a.rows - kGroup;
+    rem = nn - j;
+    pad = (uint32_t)sl | (31u << kImgShift);
+  }
+  __device__ inline uint32_t word(int kk) const {
+    const int row = (kk >> 2) < rmax ? (kk >> 2) : rmax;
+    const uint32_t w = ld_at(nbr + (size_t)row * row_words, lane_off + (uint32_t)(kk & 3) * 4u);
+    return kk < rem ? w : pad;
+  }
+};
+
+// Per-block table of the stencil offsets; codes 27..31 (the pad's 31) read zero.
+__device__ inline void eam_offsets(float4* offs, const EamArgs& a) {
   load_offsets(offs, a.hx, a.hy, a.hz);
   if (threadIdx.x >= 27 && threadIdx.x < 32) offs[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
   __syncthreads();
-  const int g = xcd_block() * blockDim.x + threadIdx.x;
-  const int s = g / LANES, j = g % LANES;
-  const int n = a.n;
-  const bool live = s < n;  // no early exit: all lanes take part in the shuffles
-  const int sl = live ? s : 0;  // a valid slot for the unconditional loads
-  const float4 qi = a.q[sl];
-  const int nn = live ? a.nnbr[s] : 0;
-  int err = 0;
-  QRange qe;
-  float rho = 0.f;  // positive terms: a plain FP32 partial per lane
-  for (int b0 = j; b0 < nn; b0 += LANES * kPre) {
-    // all list words of this stretch in flight at once
-    uint32_t ew[kPre];
+}
+
+// Density terms of CHUNK list entries (ok: a live entry, not pad).
+template <int CHUNK>
+__device__ inline void density_terms(const EamArgs& a, const float4* offs, const float4& qi,
+                                     const uint32_t (&e)[CHUNK], const bool (&ok)[CHUNK],
+                                     float& rho, QRange& qe) {
+  float4 qj[CHUNK];
 #pragma unroll
-    for (int t = 0; t < kPre; t++) {
-      const int k = b0 + t * LANES;
-      // unconditional load (a repeat of the last entry past the end)
-      const uint32_t w = a.nbr[nidx(sl, k < nn ? k : (nn > 0 ? nn - 1 : 0), n)];
-      ew[t] = (k < nn) ? w : kNone;
-    }
+  for (int u = 0; u < CHUNK; u++) qj[u] = ld_at(a.q, (e[u] & kSlotMask) << 4);
+  float rinv[CHUNK], dx[CHUNK];
+  bool in[CHUNK];
+  int seg[CHUNK];
 #pragma unroll
-    for (int t0 = 0; t0 < kPre; t0 += CHUNK) {
-      if (b0 + t0 * LANES >= nn) break;
-      uint32_t e[CHUNK];
-#pragma unroll
-      for (int u = 0; u < CHUNK; u++) e[u] = ew[t0 + u];
-      float4 qj[CHUNK];
-#pragma unroll
-      for (int u = 0; u < CHUNK; u++) qj[u] = a.q[e[u] != kNone ? (e[u] & kSlotMask) : sl];
-      float r[CHUNK], dx[CHUNK];
-      bool in[CHUNK];
-      int seg[CHUNK];
-#pragma unroll
-      for (int u = 0; u < CHUNK; u++) {
-        const float4 o = offs[e[u] >> kImgShift];
-        const float ex = qi.x - qj[u].x - o.x;
-        const float ey = qi.y - qj[u].y - o.y;
-        const float ez = qi.z - qj[u].z - o.z;
-        const float r2 = ex * ex + ey * ey + ez * ez;
-        in[u] = (e[u] != kNone) && (r2 <= a.cut_e2);
-        r[u] = in[u] ? r2 * __builtin_amdgcn_rsqf(r2) : a.edens.r0;
-        seg[u] = spline_locate32_q(a.edens, r[u], dx[u], qe);
-      }
-      float4 c[CHUNK];
-#pragma unroll
-      for (int u = 0; u < CHUNK; u++) c[u] = a.edens.c32[seg[u]];
-#pragma unroll
-      for (int u = 0; u < CHUNK; u++) {
-        float v = c[u].w + dx[u] * (c[u].z + dx[u] * (c[u].y + dx[u] * c[u].x));
-        if (a.edens.rmult) v /= r[u];
-        rho += in[u] ? v : 0.f;
-      }
-    }
+  for (int u = 0; u < CHUNK; u++) {
+    const float4 o = offs[e[u] >> kImgShift];
+    const float ex = qi.x - qj[u].x - o.x;
+    const float ey = qi.y - qj[u].y - o.y;
+    const float ez = qi.z - qj[u].z - o.z;
+    const float r2 = ex * ex + ey * ey + ez * ez;
+    in[u] = ok[u] && (r2 <= a.cut_e2);
+    rinv[u] = __builtin_amdgcn_rsqf(r2);  // inf for the pad's r2 = 0, masked below
+    seg[u] = spline_locate32_m(a.edens, in[u] ? r2 * rinv[u] : a.edens.r0, dx[u], qe);
   }
-  err |= range_flags(a.edens, qe);
+  float4 c[CHUNK];
+#pragma unroll
+  for (int u = 0; u < CHUNK; u++)
+    c[u] = in[u] ? ld_at(a.edens.c32, (uint32_t)seg[u] << 4) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int u = 0; u < CHUNK; u++) {
+    float v = c[u].w + dx[u] * (c[u].z + dx[u] * (c[u].y + dx[u] * c[u].x));
+    if (a.edens.rmult) v *= rinv[u];
+    rho += in[u] ? v : 0.f;
+  }
+}
+
+// rho_i over the lanes, then F'(rho) and F(rho) in FP64 once per atom.
+template <bool ENERGY, int LANES>
+__device__ inline void density_store(const EamArgs& a, int s, int j, bool live, const float4& qi,
+                                     float rho, int err) {
   if (err) atomicOr(a.err, err);
   const double rs = lanes_sum<LANES>((double)rho);
   if (live && j == 0) {
     a.rho[s] = (float)rs;
-    // embedding function and derivative in FP64, once per atom
     double F, dF;
     spline_read64(a.embed, (double)(float)rs, F, dF, a.err);
     a.qf[s] = make_float4(qi.x, qi.y, qi.z, (float)dF);
@@ -159,113 +180,112 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
   }
 }
 
-template <bool ENERGY, bool PACKED, int LANES, int CHUNK>
-__global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
-  constexpr int kPre = pre_words<LANES, CHUNK>();
-  __shared__ float4 offs[32];  // codes 27..31 (the empty entry's 31) read zero
-  load_offsets(offs, a.hx, a.hy, a.hz);
-  if (threadIdx.x >= 27 && threadIdx.x < 32) offs[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
-  __syncthreads();
-  const int g = xcd_block() * blockDim.x + threadIdx.x;
-  const int s = g / LANES, j = g % LANES;
-  const int n = a.n;
-  const bool live = s < n;
-  const int sl = live ? s : 0;  // a valid slot for the unconditional loads
-  const float4 qi = a.qf[sl];
-  const float fpi = qi.w;
-  const int nn = live ? a.nnbr[s] : 0;
-  int err = 0;
-  QRange qp, qe;
-  Kahan fx, fy, fz, pot, vir;
-  for (int b0 = j; b0 < nn; b0 += LANES * kPre) {
-    // all list words of this stretch in flight at once
-    uint32_t ew[kPre];
+// FP64 partials of the force pass: each step of entries is summed in FP32
+// and added in FP64 (two operations per term instead of a compensated
+// sum's four).
+struct ForceAcc {
+  double fx = 0.0, fy = 0.0, fz = 0.0, pot = 0.0, vir = 0.0;
+};
+
+// Force terms of CHUNK list entries (ok: a live entry, not pad).
+template <bool ENERGY, bool PACKED, int CHUNK>
+__device__ inline void force_terms(const EamArgs& a, const float4* offs, const float4& qi,
+                                   const uint32_t (&e)[CHUNK], const bool (&ok)[CHUNK],
+                                   ForceAcc& acc, QRange& qp, QRange& qe) {
+  float4 qj[CHUNK];
 #pragma unroll
-    for (int t = 0; t < kPre; t++) {
-      const int k = b0 + t * LANES;
-      // unconditional load (a repeat of the last entry past the end)
-      const uint32_t w = a.nbr[nidx(sl, k < nn ? k : (nn > 0 ? nn - 1 : 0), n)];
-      ew[t] = (k < nn) ? w : kNone;
-    }
+  for (int u = 0; u < CHUNK; u++) qj[u] = ld_at(a.qf, (e[u] & kSlotMask) << 4);
+  float d[CHUNK][3], r2[CHUNK], rinv[CHUNK], dxp[CHUNK], dxe[CHUNK];
+  bool in[CHUNK], ine[CHUNK];
+  int sp[CHUNK], se[CHUNK];
 #pragma unroll
-    for (int t0 = 0; t0 < kPre; t0 += CHUNK) {
-      if (b0 + t0 * LANES >= nn) break;
-      uint32_t e[CHUNK];
-#pragma unroll
-      for (int u = 0; u < CHUNK; u++) e[u] = ew[t0 + u];
-      float4 qj[CHUNK];
-#pragma unroll
-      for (int u = 0; u < CHUNK; u++) qj[u] = a.qf[e[u] != kNone ? (e[u] & kSlotMask) : sl];
-      float d[CHUNK][3], r2[CHUNK], rinv[CHUNK], dxp[CHUNK], dxe[CHUNK];
-      bool in[CHUNK], ine[CHUNK];
-      int sp[CHUNK], se[CHUNK];
-#pragma unroll
-      for (int u = 0; u < CHUNK; u++) {
-        const float4 o = offs[e[u] >> kImgShift];
-        d[u][0] = qi.x - qj[u].x - o.x;
-        d[u][1] = qi.y - qj[u].y - o.y;
-        d[u][2] = qi.z - qj[u].z - o.z;
-        r2[u] = d[u][0] * d[u][0] + d[u][1] * d[u][1] + d[u][2] * d[u][2];
-        in[u] = (e[u] != kNone) && (r2[u] <= a.cut_p2);
-        ine[u] = in[u] && (r2[u] <= a.cut_e2);
-        rinv[u] = in[u] ? __builtin_amdgcn_rsqf(r2[u]) : 1.f;
-        const float r = in[u] ? r2[u] * rinv[u] : a.phi.r0;
-        sp[u] = spline_locate32_q(a.phi, r, dxp[u], qp);
-        if (PACKED) {
-          se[u] = sp[u];
-          dxe[u] = dxp[u];
-        } else {
-          se[u] = spline_locate32_q(a.edens, ine[u] ? r : a.edens.r0, dxe[u], qe);
-        }
-      }
-      float4 cp[CHUNK], ce[CHUNK];
-#pragma unroll
-      for (int u = 0; u < CHUNK; u++) {
-        if (PACKED) {
-          cp[u] = a.pack[2 * sp[u]];
-          ce[u] = a.pack[2 * sp[u] + 1];
-        } else {
-          cp[u] = a.phi.c32[sp[u]];
-          ce[u] = a.edens.c32[se[u]];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < CHUNK; u++) {
-        const float x = dxp[u];
-        float p = cp[u].w + x * (cp[u].z + x * (cp[u].y + x * cp[u].x));
-        float dp = cp[u].z + x * (2.0f * cp[u].y + 3.0f * x * cp[u].x);
-        if (a.phi.rmult) {
-          p *= rinv[u];
-          dp = (dp - p) * rinv[u];
-        }
-        const float y = dxe[u];
-        float drho = ce[u].z + y * (2.0f * ce[u].y + 3.0f * y * ce[u].x);
-        if (a.edens.rmult) {
-          const float v = (ce[u].w + y * (ce[u].z + y * (ce[u].y + y * ce[u].x))) * rinv[u];
-          drho = (drho - v) * rinv[u];
-        }
-        drho = ine[u] ? drho : 0.f;  // GetRhoDeriv beyond the EDENS range (team.cpp:130)
-        const float fr = in[u] ? -((fpi + qj[u].w) * drho + dp) * rinv[u] : 0.f;
-        fx.add(fr * d[u][0]);
-        fy.add(fr * d[u][1]);
-        fz.add(fr * d[u][2]);
-        if (ENERGY) {
-          pot.add(in[u] ? 0.5f * p : 0.f);
-          vir.add(fr * r2[u]);
-        }
-      }
+  for (int u = 0; u < CHUNK; u++) {
+    const float4 o = offs[e[u] >> kImgShift];
+    d[u][0] = qi.x - qj[u].x - o.x;
+    d[u][1] = qi.y - qj[u].y - o.y;
+    d[u][2] = qi.z - qj[u].z - o.z;
+    r2[u] = d[u][0] * d[u][0] + d[u][1] * d[u][1] + d[u][2] * d[u][2];
+    in[u] = ok[u] && (r2[u] <= a.cut_p2);
+    ine[u] = in[u] && (r2[u] <= a.cut_e2);
+    // every use of rinv outside the cut (inf for the pad's r2 = 0) is
+    // masked below
+    rinv[u] = __builtin_amdgcn_rsqf(r2[u]);
+    const float r = in[u] ? r2[u] * rinv[u] : a.phi.r0;
+    sp[u] = spline_locate32_m(a.phi, r, dxp[u], qp);
+    if (PACKED) {
+      se[u] = sp[u];
+      dxe[u] = dxp[u];
+    } else {
+      se[u] = spline_locate32_m(a.edens, ine[u] ? r : a.edens.r0, dxe[u], qe);
     }
   }
-  err |= range_flags(a.phi, qp);
+  float4 cp[CHUNK], ce[CHUNK];
+#pragma unroll
+  for (int u = 0; u < CHUNK; u++) {
+    if (PACKED) {
+      if (in[u]) {
+        cp[u] = ld_at(a.pack, (uint32_t)sp[u] << 5);
+        ce[u] = ld_at(a.pack, ((uint32_t)sp[u] << 5) + 16u);
+      } else {
+        cp[u] = ce[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    } else {
+      const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+      cp[u] = in[u] ? ld_at(a.phi.c32, (uint32_t)sp[u] << 4) : z;
+      ce[u] = ine[u] ? ld_at(a.edens.c32, (uint32_t)se[u] << 4) : z;
+    }
+  }
+  float cx = 0.f, cy = 0.f, cz = 0.f, cpot = 0.f, cvir = 0.f;
+#pragma unroll
+  for (int u = 0; u < CHUNK; u++) {
+    const float x = dxp[u];
+    float p = cp[u].w + x * (cp[u].z + x * (cp[u].y + x * cp[u].x));
+    float dp = cp[u].z + x * (2.0f * cp[u].y + 3.0f * x * cp[u].x);
+    if (a.phi.rmult) {
+      p *= rinv[u];
+      dp = (dp - p) * rinv[u];
+    }
+    const float y = dxe[u];
+    float drho = ce[u].z + y * (2.0f * ce[u].y + 3.0f * y * ce[u].x);
+    if (a.edens.rmult) {
+      const float v = (ce[u].w + y * (ce[u].z + y * (ce[u].y + y * ce[u].x))) * rinv[u];
+      drho = (drho - v) * rinv[u];
+    }
+    drho = ine[u] ? drho : 0.f;  // GetRhoDeriv beyond the EDENS range (team.cpp:130)
+    const float fr = in[u] ? -((qi.w + qj[u].w) * drho + dp) * rinv[u] : 0.f;
+    cx += fr * d[u][0];
+    cy += fr * d[u][1];
+    cz += fr * d[u][2];
+    if (ENERGY) {
+      cpot += in[u] ? 0.5f * p : 0.f;
+      cvir += fr * r2[u];
+    }
+  }
+  acc.fx += (double)cx;
+  acc.fy += (double)cy;
+  acc.fz += (double)cz;
+  if (ENERGY) {
+    acc.pot += (double)cpot;
+    acc.vir += (double)cvir;
+  }
+}
+
+// Forces (and per-atom energy and virial) over the lanes.
+template <bool ENERGY, bool PACKED, int LANES>
+__device__ inline void force_store(const EamArgs& a, int s, int j, bool live, const ForceAcc& acc,
+                                   const QRange& qp, const QRange& qe) {
+  int err = range_flags(a.phi, qp);
   if (!PACKED) err |= range_flags(a.edens, qe);
   if (err) atomicOr(a.err, err);
-  const double sx = lanes_sum<LANES>(fx), sy = lanes_sum<LANES>(fy), sz = lanes_sum<LANES>(fz);
+  const double sx = lanes_sum<LANES>(acc.fx), sy = lanes_sum<LANES>(acc.fy),
+               sz = lanes_sum<LANES>(acc.fz);
   double sp = 0.0, sv = 0.0;
   if (ENERGY) {
-    sp = lanes_sum<LANES>(pot);
-    sv = lanes_sum<LANES>(vir);
+    sp = lanes_sum<LANES>(acc.pot);
+    sv = lanes_sum<LANES>(acc.vir);
   }
   if (live && j == 0) {
+    const int n = a.n;
     a.f[s] = (float)sx;
     a.f[(size_t)n + s] = (float)sy;
     a.f[2 * (size_t)n + s] = (float)sz;
@@ -274,6 +294,72 @@ __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
       a.vir[s] = (float)sv;
     }
   }
+}
+
+template <bool ENERGY, int LANES, int CHUNK>
+__global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
+  constexpr int kPre = pre_words<LANES, CHUNK>();
+  __shared__ float4 offs[32];
+  eam_offsets(offs, a);
+  const int g = xcd_block() * blockDim.x + threadIdx.x;
+  const int s = g / LANES, j = g % LANES;
+  const bool live = s < a.n;  // no early exit: all lanes take part in the shuffles
+  const int sl = live ? s : 0;  // a valid slot for the unconditional loads
+  const float4 qi = a.q[sl];
+  const ListLanes<LANES> ll(a, sl, j, live ? a.nnbr[s] : 0);
+  QRange qe;
+  float rho = 0.f;  // positive terms: a plain FP32 partial per lane
+  for (int kk0 = 0; kk0 < ll.rem; kk0 += LANES * kPre) {
+    uint32_t ew[kPre];
+#pragma unroll
+    for (int t = 0; t < kPre; t++) ew[t] = ll.word(kk0 + t * LANES);
+#pragma unroll
+    for (int t0 = 0; t0 < kPre; t0 += CHUNK) {
+      if (kk0 + t0 * LANES >= ll.rem) break;
+      uint32_t e[CHUNK];
+      bool ok[CHUNK];
+#pragma unroll
+      for (int u = 0; u < CHUNK; u++) {
+        e[u] = ew[t0 + u];
+        ok[u] = kk0 + (t0 + u) * LANES < ll.rem;
+      }
+      density_terms<CHUNK>(a, offs, qi, e, ok, rho, qe);
+    }
+  }
+  density_store<ENERGY, LANES>(a, s, j, live, qi, rho, range_flags(a.edens, qe));
+}
+
+template <bool ENERGY, bool PACKED, int LANES, int CHUNK>
+__global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
+  constexpr int kPre = pre_words<LANES, CHUNK>();
+  __shared__ float4 offs[32];
+  eam_offsets(offs, a);
+  const int g = xcd_block() * blockDim.x + threadIdx.x;
+  const int s = g / LANES, j = g % LANES;
+  const bool live = s < a.n;
+  const int sl = live ? s : 0;
+  const float4 qi = a.qf[sl];
+  const ListLanes<LANES> ll(a, sl, j, live ? a.nnbr[s] : 0);
+  QRange qp, qe;
+  ForceAcc acc;
+  for (int kk0 = 0; kk0 < ll.rem; kk0 += LANES * kPre) {
+    uint32_t ew[kPre];
+#pragma unroll
+    for (int t = 0; t < kPre; t++) ew[t] = ll.word(kk0 + t * LANES);
+#pragma unroll
+    for (int t0 = 0; t0 < kPre; t0 += CHUNK) {
+      if (kk0 + t0 * LANES >= ll.rem) break;
+      uint32_t e[CHUNK];
+      bool ok[CHUNK];
+#pragma unroll
+      for (int u = 0; u < CHUNK; u++) {
+        e[u] = ew[t0 + u];
+        ok[u] = kk0 + (t0 + u) * LANES < ll.rem;
+      }
+      force_terms<ENERGY, PACKED, CHUNK>(a, offs, qi, e, ok, acc, qp, qe);
+    }
+  }
+  force_store<ENERGY, PACKED, LANES>(a, s, j, live, acc, qp, qe);
 }
 
 // Interleave the PAIR and EDENS coefficients when they share one grid.
@@ -299,7 +385,7 @@ void build_pack(Ctx& c, int tp, int te) {
 struct Shape {
   int lanes, chunk;
 };
-constexpr Shape kShapes[] = {{4, 2}, {4, 4}, {8, 1}, {8, 2}, {2, 4}, {16, 1}};
+constexpr Shape kShapes[] = {{4, 2}, {4, 4}, {8, 1}, {8, 2}, {2, 4}, {16, 1}, {4, 1}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 template <int L, int C>
@@ -312,25 +398,33 @@ void launch_shape(Ctx& c, const EamArgs& a, bool energy, bool pass1, bool pass2)
   }
   if (pass2) {
     TimerScope ts(c, TG_FORCE);
-    if (a.pack) {
-      if (energy) k_eam_force<true, true, L, C><<<nb, kBlock, 0, c.stream>>>(a);
-      else k_eam_force<false, true, L, C><<<nb, kBlock, 0, c.stream>>>(a);
-    } else {
-      if (energy) k_eam_force<true, false, L, C><<<nb, kBlock, 0, c.stream>>>(a);
-      else k_eam_force<false, false, L, C><<<nb, kBlock, 0, c.stream>>>(a);
-    }
+    const bool pk = a.pack != nullptr;
+    if (pk && energy) k_eam_force<true, true, L, C><<<nb, kBlock, 0, c.stream>>>(a);
+    else if (pk) k_eam_force<false, true, L, C><<<nb, kBlock, 0, c.stream>>>(a);
+    else if (energy) k_eam_force<true, false, L, C><<<nb, kBlock, 0, c.stream>>>(a);
+    else k_eam_force<false, false, L, C><<<nb, kBlock, 0, c.stream>>>(a);
   }
+}
+
+// A shape reads its list rows in groups of max(1, lanes / 4) with a 32-bit
+// lane offset: the row count must be a multiple of the group and the offset
+// within 4 GiB.  The default shape (4 lanes) fits every list.
+bool shape_fits(int shape, int n, int rows) {
+  const int group = kShapes[shape].lanes >= 4 ? kShapes[shape].lanes / 4 : 1;
+  return rows >= group && rows % group == 0 && (uint64_t)group * (uint64_t)n * 16u <= 0xFFFFFFFFull;
 }
 
 void launch(Ctx& c, const EamArgs& a, bool energy, int shape, bool pass1 = true,
             bool pass2 = true) {
+  if (!shape_fits(shape, a.n, a.rows)) shape = 0;
   switch (shape) {
     case 0: launch_shape<4, 2>(c, a, energy, pass1, pass2); break;
     case 1: launch_shape<4, 4>(c, a, energy, pass1, pass2); break;
     case 2: launch_shape<8, 1>(c, a, energy, pass1, pass2); break;
     case 3: launch_shape<8, 2>(c, a, energy, pass1, pass2); break;
     case 4: launch_shape<2, 4>(c, a, energy, pass1, pass2); break;
-    default: launch_shape<16, 1>(c, a, energy, pass1, pass2); break;
+    case 5: launch_shape<16, 1>(c, a, energy, pass1, pass2); break;
+    default: launch_shape<4, 1>(c, a, energy, pass1, pass2); break;
   }
 }
 
@@ -379,6 +473,8 @@ void eam_forces(Ctx& c, bool energy) {
   a.vir = c.vir32;
   a.err = c.dstat + ST_ERR;
   a.n = c.n;
+  a.rows = c.maxnbr / 4;
+  if (a.rows < 1) throw ApiFail(OMDG_EINVAL, "EAM forces need a neighbour list");
   const int tp = c.tab_index[OMDG_TAB_PAIR][0][0];
   const int te = c.tab_index[OMDG_TAB_EDENS][0][0];
   a.phi = tab_view(c, tp);
@@ -396,8 +492,15 @@ void eam_forces(Ctx& c, bool energy) {
   a.hy = (float)c.grid.h[1];
   a.hz = (float)c.grid.h[2];
   if (c.eam_shape[0] < 0) {
-    // small systems keep the default; large ones time the candidates
-    if (c.n >= 16384) {
+    // OMDG_EAM_SHAPE=<index into kShapes> fixes the shape of both passes;
+    // otherwise small systems keep the default and large ones time the
+    // candidates
+    const char* fixed = std::getenv("OMDG_EAM_SHAPE");
+    if (fixed && *fixed) {
+      const int v = std::atoi(fixed);
+      if (v < 0 || v >= kNumShapes) throw ApiFail(OMDG_EINVAL, "OMDG_EAM_SHAPE out of range");
+      c.eam_shape[0] = c.eam_shape[1] = v;
+    } else if (c.n >= 16384) {
       launch(c, a, false, 0, true, false);  // F'(rho) for the force-pass tuning
       c.eam_shape[0] = tune_pass(c, a, true);
       c.eam_shape[1] = tune_pass(c, a, false);

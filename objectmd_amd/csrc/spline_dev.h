@@ -169,6 +169,19 @@ __device__ inline int spline_locate32_q(const TabView& t, float r, float& dx, QR
   return x;
 }
 
+// spline_locate32_q with the top segment folded into the clamp: q in
+// [n - 1, n) is clamped to n - 1.5, giving x = n - 2 and dx = rval - (n - 2) dr,
+// the point spline_locate32_q reaches as x = n - 1 moved down one segment.
+// Three fewer operations per lookup; the range check is unchanged.
+__device__ inline int spline_locate32_m(const TabView& t, float r, float& dx, QRange& qr) {
+  const float rval = r - t.r0;
+  const float q = rval * t.inv_dr;
+  qr.add(q);
+  const int x = (int)__builtin_amdgcn_fmed3f(q, -1.5f, (float)t.n - 1.5f);  // in [-1, n-2]
+  dx = rval - (float)x * t.dr;
+  return x < 0 ? 0 : x;
+}
+
 // Compensated (Kahan) accumulation of FP32 terms.
 struct Kahan {
   float s = 0.f, c = 0.f;

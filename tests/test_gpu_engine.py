@@ -144,6 +144,31 @@ def test_eam_forces_vs_oracle(cells):
     eng.close()
 
 
+# Every shape of the two EAM passes (lanes per atom, entries in flight),
+# with the automatic list capacity and with capacities whose row count
+# (capacity / 4) is not a multiple of a shape's row group of lanes / 4 (those
+# shapes then run the default one).
+@pytest.mark.parametrize("maxnbr", [0, 68, 72])
+@pytest.mark.parametrize("shape", range(7))
+def test_eam_shapes_vs_oracle(shape, maxnbr, monkeypatch):
+    monkeypatch.setenv("OMDG_EAM_SHAPE", str(shape))
+    s = eam_system(cells=10, seed=2468)
+    say(f"[eam-shape] shape={shape} maxnbr={maxnbr} n={len(s['x'])}")
+    eng = _eam_engine(s)
+    eng.set_neighbor(0.3, 10, maxnbr)
+    ep, vir = eng.compute()
+    f = eng.download(om.F)
+    rho = eng.download(om.RHO)
+    fo, poto, viro, rhoo = _oracle_eam(s["x"], s, 5.5 + 0.3)
+    err = rel_rms(f, fo)
+    say(f"[eam-shape] force relRMS={err:.3e}")
+    assert err <= FORCE_RTOL
+    assert ep == pytest.approx(poto.sum(), rel=1e-6)
+    assert abs(vir - viro.sum() / 2) <= 2e-5 * np.abs(viro).sum()
+    assert rel_rms(rho, rhoo) < 1e-6
+    eng.close()
+
+
 def test_eam_perfect_crystal():
     pos, (lo, hi) = lattice.fcc100(8, 8, 8, 4.05)
     s = dict(x=pos, v=np.zeros_like(pos), lo=lo, hi=hi, L=hi - lo, pbc=7,
@@ -156,9 +181,12 @@ def test_eam_perfect_crystal():
     eng.close()
 
 
-def test_eam_trajectory_vs_oracle():
+@pytest.mark.parametrize("shape", ["default", "5"])
+def test_eam_trajectory_vs_oracle(shape, monkeypatch):
+    if shape != "default":
+        monkeypatch.setenv("OMDG_EAM_SHAPE", shape)
     s = eam_system(cells=5, seed=777)
-    say(f"[eam-traj] n={len(s['x'])}")
+    say(f"[eam-traj] n={len(s['x'])} shape={shape}")
     eng = _eam_engine(s, skin=0.4)
     ep0, _ = eng.compute()
     eng.zero_forces()                       # FirstRun (system.cpp:872)
