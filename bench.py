@@ -179,7 +179,7 @@ def bench_eam(args):
             "workload": "EAM Al fcc a=4.05 A, 40^3 cells = 256,000 atoms, PBC, 300 K + "
                         "0.02 A displacements, Johnson-type EAM 5,000-point splines, "
                         "rc 5.5 A, dt 2 fs",
-            "atoms": n, "force_dtype": "f32 with compensated per-atom sums",
+            "atoms": n, "force_dtype": "f32 pair terms, f64 per-atom sums",
             "integrator_dtype": "f64", "force_rel_rms_vs_fp64": force_err,
             "energy_per_atom_start_end_eV": [e_start, (ek + ep - base) / n],
             "skin_A": skin,

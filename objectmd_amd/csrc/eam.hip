@@ -60,7 +60,6 @@ struct EamArgs {
   float* vir;
   int* err;
   int n;
-  int rows;  // list rows (capacity / 4)
   float cut_e2, cut_p2;
   float hx, hy, hz;
   TabView phi, edens, embed;
@@ -95,32 +94,27 @@ __device__ inline T ld_at(const T* base, uint32_t byte_off) {
 // kk + j (kk = kk0 + t * LANES) lies in row (kk >> 2) + (j >> 2), word
 // (kk & 3) + (j & 3) of the blocked layout of nidx, since (kk & 3) + (j & 3)
 // stays below 4 for LANES = 2 and kk & 3 = 0 for LANES >= 4.  The row is a
-// uniform pointer, the lane part a 32-bit offset fixed for the kernel.  Rows
-// are clamped to the last whole group of LANES / 4 rows of the list (the
-// list's row count is a multiple of the group, checked at launch), so reads
-// past the end stay inside it; entries at or past the atom's count become
-// `pad`, the atom itself with stencil code 31 (zero offset).
+// uniform pointer, the lane part a 32-bit offset fixed for the kernel (its
+// range is checked at launch).  Entries at or past the atom's count are not
+// read (the load is masked off, so no row past the list is touched) and
+// become `pad`, the atom itself with stencil code 31 (zero offset).
 template <int LANES>
 struct ListLanes {
-  static constexpr int kGroup = LANES >= 4 ? LANES / 4 : 1;  // rows one stretch step spans
   const uint32_t* nbr;
-  size_t row_words;  // 4 n
+  size_t row_words;   // 4 n
   uint32_t lane_off;  // bytes: ((j >> 2) n + s) * 16 + (j & 3) * 4
-  int rmax;           // largest first row
   int rem;            // entries of this lane: kk < rem
   uint32_t pad;
   __device__ ListLanes(const EamArgs& a, int sl, int j, int nn) {
     nbr = a.nbr;
     row_words = (size_t)a.n * 4;
     lane_off = ((uint32_t)(j >> 2) * (uint32_t)a.n + (uint32_t)sl) * 16u + (uint32_t)(j & 3) * 4u;
-    rmax = a.rows - kGroup;
     rem = nn - j;
     pad = (uint32_t)sl | (31u << kImgShift);
   }
   __device__ inline uint32_t word(int kk) const {
-    const int row = (kk >> 2) < rmax ? (kk >> 2) : rmax;
-    const uint32_t w = ld_at(nbr + (size_t)row * row_words, lane_off + (uint32_t)(kk & 3) * 4u);
-    return kk < rem ? w : pad;
+    return kk < rem ? ld_at(nbr + (size_t)(kk >> 2) * row_words, lane_off + (uint32_t)(kk & 3) * 4u)
+                    : pad;
   }
 };
 
@@ -406,17 +400,17 @@ void launch_shape(Ctx& c, const EamArgs& a, bool energy, bool pass1, bool pass2)
   }
 }
 
-// A shape reads its list rows in groups of max(1, lanes / 4) with a 32-bit
-// lane offset: the row count must be a multiple of the group and the offset
-// within 4 GiB.  The default shape (4 lanes) fits every list.
-bool shape_fits(int shape, int n, int rows) {
+// A shape's lanes read list words at a 32-bit offset of up to
+// max(1, lanes / 4) * 16 n bytes; the default shape (4 lanes) fits every
+// system (n < 2^27).
+bool shape_fits(int shape, int n) {
   const int group = kShapes[shape].lanes >= 4 ? kShapes[shape].lanes / 4 : 1;
-  return rows >= group && rows % group == 0 && (uint64_t)group * (uint64_t)n * 16u <= 0xFFFFFFFFull;
+  return (uint64_t)group * (uint64_t)n * 16u <= 0xFFFFFFFFull;
 }
 
 void launch(Ctx& c, const EamArgs& a, bool energy, int shape, bool pass1 = true,
             bool pass2 = true) {
-  if (!shape_fits(shape, a.n, a.rows)) shape = 0;
+  if (!shape_fits(shape, a.n)) shape = 0;
   switch (shape) {
     case 0: launch_shape<4, 2>(c, a, energy, pass1, pass2); break;
     case 1: launch_shape<4, 4>(c, a, energy, pass1, pass2); break;
@@ -473,8 +467,6 @@ void eam_forces(Ctx& c, bool energy) {
   a.vir = c.vir32;
   a.err = c.dstat + ST_ERR;
   a.n = c.n;
-  a.rows = c.maxnbr / 4;
-  if (a.rows < 1) throw ApiFail(OMDG_EINVAL, "EAM forces need a neighbour list");
   const int tp = c.tab_index[OMDG_TAB_PAIR][0][0];
   const int te = c.tab_index[OMDG_TAB_EDENS][0][0];
   a.phi = tab_view(c, tp);

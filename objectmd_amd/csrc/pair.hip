@@ -49,60 +49,96 @@ __device__ inline double shifted(double x, int s, double L) {
 
 // lanes consecutive threads share one atom (a power of two up to 64): lane j
 // takes list entries j, j + lanes, ...; the partial sums meet by shuffles.
-template <bool ENERGY>
+// A lane requests the list words of U entries, then their positions and
+// types, then their table records, before using any (three memory round
+// trips per U entries); the sums run in list order as one entry at a time.
+// ONE: a single atom type, whose table and cut are wavefront-uniform (no
+// type gathers, no per-lane lookups of the table descriptors).
+template <bool ENERGY, int U, bool ONE>
 __global__ void __launch_bounds__(kBlock) k_pair64(PairArgs a, int lanes) {
   const int g = xcd_block() * blockDim.x + threadIdx.x;
   const int s = g / lanes, sub = g % lanes;  // lanes | 64: a group never spans waves
   const bool live = s < a.n;
   const int n = a.n;
   double fx = 0, fy = 0, fz = 0, pot = 0, vir = 0;
+  int err = 0;
   if (live) {
     const double xi = a.x[s], yi = a.x[(size_t)n + s], zi = a.x[2 * (size_t)n + s];
     const int c = a.cellid[s];
     const int k[3] = {c % a.M[0], (c / a.M[0]) % a.M[1], c / (a.M[0] * a.M[1])};
-    const int ti = a.type[s];
+    const int ti = ONE ? 0 : a.type[s];
     const int nn = a.nnbr[s];
-    for (int e = sub; e < nn; e += lanes) {
-      const uint32_t w = a.nbr[nidx(s, e, n)];
-      const int t = (int)(w & kSlotMask);
-      const int img = (int)(w >> kImgShift);
-      const int o[3] = {img % 3 - 1, (img / 3) % 3 - 1, img / 9 - 1};
-      int sh[3];
-      for (int d = 0; d < 3; d++) {
-        const int kk = k[d] + o[d];
-        sh[d] = ((a.pbc >> d) & 1) ? (kk < 0 ? -1 : (kk >= a.M[d] ? 1 : 0)) : 0;
+    for (int e0 = sub; e0 < nn; e0 += lanes * U) {
+      uint32_t w[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int e = e0 + u * lanes;
+        w[u] = a.nbr[nidx(s, e < nn ? e : nn - 1, n)];  // past the end: a valid entry
       }
-      const int slot = ti * kMaxTypes + a.type[t];
-      const int tb = a.tab[slot];
-      if (tb < 0) continue;
-      double dx, dy, dz, r2;
-      {
-  #pragma clang fp contract(off)
-        dx = xi - shifted(a.x[t], sh[0], a.L[0]);
-        dy = yi - shifted(a.x[(size_t)n + t], sh[1], a.L[1]);
-        dz = zi - shifted(a.x[2 * (size_t)n + t], sh[2], a.L[2]);
-        r2 = dx * dx + dy * dy + dz * dz;
+      double xj[U], yj[U], zj[U];
+      int tj[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int t = (int)(w[u] & kSlotMask);
+        xj[u] = a.x[t];
+        yj[u] = a.x[(size_t)n + t];
+        zj[u] = a.x[2 * (size_t)n + t];
+        tj[u] = ONE ? 0 : a.type[t];
       }
-      if (r2 < 1e-6) {
-        atomicOr(a.err, ERR_OVERLAP);
-        continue;
+      double d[U][3], r2[U], r[U], sdx[U];
+      int seg[U], tb[U];
+      bool in[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int img = (int)(w[u] >> kImgShift);
+        const int o[3] = {img % 3 - 1, (img / 3) % 3 - 1, img / 9 - 1};
+        int sh[3];
+        for (int dd = 0; dd < 3; dd++) {
+          const int kk = k[dd] + o[dd];
+          sh[dd] = ((a.pbc >> dd) & 1) ? (kk < 0 ? -1 : (kk >= a.M[dd] ? 1 : 0)) : 0;
+        }
+        const int slot = ONE ? 0 : ti * kMaxTypes + tj[u];
+        tb[u] = a.tab[slot];
+        {
+#pragma clang fp contract(off)
+          d[u][0] = xi - shifted(xj[u], sh[0], a.L[0]);
+          d[u][1] = yi - shifted(yj[u], sh[1], a.L[1]);
+          d[u][2] = zi - shifted(zj[u], sh[2], a.L[2]);
+          r2[u] = d[u][0] * d[u][0] + d[u][1] * d[u][1] + d[u][2] * d[u][2];
+        }
+        in[u] = (e0 + u * lanes < nn) && tb[u] >= 0;
+        if (in[u] && r2[u] < 1e-6) {
+          err |= ERR_OVERLAP;
+          in[u] = false;
+        }
+        r[u] = sqrt(r2[u]);
+        in[u] = in[u] && r[u] < a.cut[slot];
+        seg[u] = 0;
+        sdx[u] = 0.0;
+        if (in[u]) in[u] = spline_locate64_e(a.tabs[tb[u]], r[u], seg[u], sdx[u], err);
       }
-      const double r = sqrt(r2);
-      if (r < a.cut[slot]) {
+      double4 cf[U];
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        cf[u] = in[u] ? a.tabs[tb[u]].c64[seg[u]] : make_double4(0.0, 0.0, 0.0, 0.0);
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (!in[u]) continue;
         double p, dp;
-        spline_read64(a.tabs[tb], r, p, dp, a.err);
-        const double ff = -dp / r;
-        const double gx = dx * ff, gy = dy * ff, gz = dz * ff;
+        spline_eval64(a.tabs[tb[u]], cf[u], sdx[u], r[u], p, dp);
+        const double ff = -dp / r[u];
+        const double gx = d[u][0] * ff, gy = d[u][1] * ff, gz = d[u][2] * ff;
         fx += gx;
         fy += gy;
         fz += gz;
         if (ENERGY) {
           pot += 0.5 * p;
-          vir += gx * dx + gy * dy + gz * dz;
+          vir += gx * d[u][0] + gy * d[u][1] + gz * d[u][2];
         }
       }
     }
   }
+  if (err) atomicOr(a.err, err);
   for (int o = lanes / 2; o > 0; o >>= 1) {
     fx += __shfl_xor(fx, o, 64);
     fy += __shfl_xor(fy, o, 64);
@@ -224,10 +260,15 @@ void pair_forces(Ctx& c, bool energy) {
   if (c.fp64) {
     // small systems: enough lanes per atom for ~128k threads
     int lanes = 1;
-    while (lanes < 16 && (long)c.n * lanes < 131072) lanes *= 2;
+    while (lanes < 32 && (long)c.n * lanes < 131072) lanes *= 2;
     const int nbl = grid_blocks((long)c.n * lanes, kBlock);
-    if (energy) k_pair64<true><<<nbl, kBlock, 0, c.stream>>>(a, lanes);
-    else k_pair64<false><<<nbl, kBlock, 0, c.stream>>>(a, lanes);
+    if (c.ntypes == 1) {
+      if (energy) k_pair64<true, 4, true><<<nbl, kBlock, 0, c.stream>>>(a, lanes);
+      else k_pair64<false, 4, true><<<nbl, kBlock, 0, c.stream>>>(a, lanes);
+    } else {
+      if (energy) k_pair64<true, 4, false><<<nbl, kBlock, 0, c.stream>>>(a, lanes);
+      else k_pair64<false, 4, false><<<nbl, kBlock, 0, c.stream>>>(a, lanes);
+    }
   } else {
     if (energy) k_pair32<true><<<nb, kBlock, 0, c.stream>>>(a);
     else k_pair32<false><<<nb, kBlock, 0, c.stream>>>(a);

@@ -65,16 +65,18 @@ __device__ inline bool spline_value32(const TabView& t, float r, float& val, int
   return true;
 }
 
-__device__ inline bool spline_locate64(const TabView& t, double r, int& x, double& dx,
-                                       int* err) {
+// FP64 segment location (as spline_locate32) with the error bits OR'd into
+// `err`, so a kernel can raise them with one atomic per lane at its end.
+__device__ inline bool spline_locate64_e(const TabView& t, double r, int& x, double& dx,
+                                         int& err) {
   const double rval = r - t.r0d;
   const double q = rval / t.drd;
   if (!(q < (double)t.n)) {
-    atomicOr(err, ERR_RANGE_HI);
+    err |= ERR_RANGE_HI;
     return false;
   }
   if (q <= -2.0) {
-    atomicOr(err, ERR_RANGE_LO);
+    err |= ERR_RANGE_LO;
     return false;
   }
   x = (int)q;
@@ -87,6 +89,25 @@ __device__ inline bool spline_locate64(const TabView& t, double r, int& x, doubl
   return true;
 }
 
+__device__ inline bool spline_locate64(const TabView& t, double r, int& x, double& dx,
+                                       int* err) {
+  int e = 0;
+  const bool ok = spline_locate64_e(t, r, x, dx, e);
+  if (e) atomicOr(err, e);
+  return ok;
+}
+
+// Value and derivative of one segment's cubic at dx (treader.cpp:237-264).
+__device__ inline void spline_eval64(const TabView& t, const double4& c, double dx, double r,
+                                     double& val, double& dval) {
+  val = c.w + dx * (c.z + dx * (c.y + dx * c.x));
+  dval = c.z + dx * (2.0 * c.y + 3.0 * dx * c.x);
+  if (t.rmult) {
+    val /= r;
+    dval = (dval - val) / r;
+  }
+}
+
 __device__ inline bool spline_read64(const TabView& t, double r, double& val, double& dval,
                                      int* err) {
   int x;
@@ -95,13 +116,7 @@ __device__ inline bool spline_read64(const TabView& t, double r, double& val, do
     val = dval = 0.0;
     return false;
   }
-  const double4 c = t.c64[x];
-  val = c.w + dx * (c.z + dx * (c.y + dx * c.x));
-  dval = c.z + dx * (2.0 * c.y + 3.0 * dx * c.x);
-  if (t.rmult) {
-    val /= r;
-    dval = (dval - val) / r;
-  }
+  spline_eval64(t, t.c64[x], dx, r, val, dval);
   return true;
 }
 

@@ -145,9 +145,9 @@ def test_eam_forces_vs_oracle(cells):
 
 
 # Every shape of the two EAM passes (lanes per atom, entries in flight),
-# with the automatic list capacity and with capacities whose row count
-# (capacity / 4) is not a multiple of a shape's row group of lanes / 4 (those
-# shapes then run the default one).
+# with the automatic list capacity (a multiple of 16 entries) and with
+# capacities of 68 and 72 entries, which end part-way through the 8- and
+# 16-lane groups of list rows.
 @pytest.mark.parametrize("maxnbr", [0, 68, 72])
 @pytest.mark.parametrize("shape", range(7))
 def test_eam_shapes_vs_oracle(shape, maxnbr, monkeypatch):
