@@ -21,9 +21,9 @@
 // nodes, so any number of steps is captured into one HIP graph):
 //   k_rep_dt   DynamicTimeStep per replica every `update` steps
 //              (include/modify/DynamicTimeStep.h:61-71)
-//   kick_drift (+ per-replica largest displacement since the replica's
-//              build), k_rep_need (rebuild a replica before one more step
-//              could carry an atom past skin/2), the conditional build,
+//   kick_drift (+ per-replica two largest displacements since the
+//              replica's build), k_rep_need (rebuild a replica once they sum
+//              to the skin), the conditional build,
 //              force pass, kick (+ damper, clamp, range limiter, per-replica
 //              max speed), k_rep_tick (elapsed time per replica, step count)
 #include <hip/hip_runtime.h>
@@ -379,7 +379,7 @@ struct RepArgs {
   double mass[kMaxTypes];
   double dtmax, maxpath;
   int update;
-  float half_skin2;  // (skin/2)^2, less a rounding margin
+  float skin_m;  // skin, less a rounding margin
 };
 
 __device__ inline double block_reduce(double v, bool is_max) {
@@ -432,16 +432,21 @@ __global__ void k_rep_dt(RepArgs a) {
   a.dt[r] = nd;
 }
 
-// Rebuild replica r in this step if one of its atoms has moved skin/2 or
-// more since the replica's last build.  The check runs after every drift,
-// before the force pass, so every pair inside the cut is on the list the
-// forces use.  The displacement maximum is cleared for the next drift.
+// Rebuild replica r in this step if the two largest displacements of its
+// atoms since the replica's last build sum to the skin or more (a single
+// fast atom, a sputtered or reflected one, may then travel almost the whole
+// skin between builds instead of half of it).  The check runs after every
+// drift, before the force pass, so every pair inside the cut is on the list
+// the forces use.  The displacement maxima are cleared for the next drift.
 __global__ void k_rep_need(RepArgs a) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.nrep) return;
-  const float d2 = __uint_as_float(a.d2[r]);
-  a.d2[r] = 0u;
-  const int nd = d2 >= a.half_skin2 ? 1 : 0;
+  // no pair can have closed by more than the sum of the two largest
+  // displacements since the build: rebuild once that sum reaches the skin
+  const float d1 = __uint_as_float(a.d2[2 * r]), d2 = __uint_as_float(a.d2[2 * r + 1]);
+  a.d2[2 * r] = 0u;
+  a.d2[2 * r + 1] = 0u;
+  const int nd = sqrtf(d1) + sqrtf(d2) >= a.skin_m ? 1 : 0;
   a.need[r] = nd;
   if (nd) atomicAdd(a.stat + ST_REBUILDS, 1);
 }
@@ -487,7 +492,7 @@ RepArgs rep_args(Ctx& c) {
   a.need = c.rep_need;
   a.step = c.dstep;
   a.stat = c.dstat;
-  a.half_skin2 = (float)(0.25 * c.skin * c.skin * (1.0 - 1e-4));
+  a.skin_m = (float)(c.skin * (1.0 - 1e-4));
   a.n = c.n;
   a.nrep = c.nrep;
   a.has_flags = c.has_flags;

@@ -52,9 +52,20 @@ struct IntArgs {
   const double* rep_dt;
   const RepGrid* rep_grid;
   unsigned long long* rep_v2;  // per-replica max |v|^2 (double bits)
-  unsigned int* rep_d2;        // per-replica max displacement^2 (float bits)
+  unsigned int* rep_d2;        // per replica: two largest displacements^2 (float bits)
   int first;                   // FirstRun: modifies only, no velocity change
 };
+
+// Insert a wavefront's largest value v1 and its runner-up v2 (v2 <= v1,
+// from another lane) into the running {largest, second largest} pair p of
+// non-negative floats kept as bits: the value v1 displaces from p[0] (or v1
+// itself) competes for p[1].  Every value reaching p[1] is the smaller of two
+// distinct inserted values, and the true second largest always reaches it.
+__device__ inline void top2_insert(unsigned int* p, float v1, float v2) {
+  const unsigned int old = atomicMax(p, __float_as_uint(v1));
+  atomicMax(p + 1, old < __float_as_uint(v1) ? old : __float_as_uint(v1));
+  if (v2 > 0.f) atomicMax(p + 1, __float_as_uint(v2));
+}
 
 __device__ inline float wave_max(float v) {
   for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
@@ -185,14 +196,18 @@ __global__ void __launch_bounds__(kBlock) k_kick_drift(IntArgs a) {
     if (!move) disp2 = 0.f;
   }
   if (BATCH) {
-    // per-replica maximum for the device-side rebuild decision
+    // the two largest displacements per replica for the device-side rebuild
+    // decision (one wavefront reduction per replica-uniform wavefront)
     const int r = s < a.n ? a.rep[s] : -1;
     const int r0 = __shfl(r, 0, 64);
     if (__all(r == r0)) {
-      const float m = wave_max(disp2);
-      if ((threadIdx.x & 63) == 0 && r0 >= 0 && m > 0.f) atomicMax(a.rep_d2 + r0, __float_as_uint(m));
+      const float m1 = wave_max(disp2);
+      const unsigned long long at = __ballot(disp2 == m1);
+      const int first = __ffsll((long long)at) - 1;  // one lane holding m1
+      const float m2 = wave_max((int)(threadIdx.x & 63) == first ? 0.f : disp2);
+      if ((threadIdx.x & 63) == 0 && r0 >= 0 && m1 > 0.f) top2_insert(a.rep_d2 + 2 * r0, m1, m2);
     } else if (r >= 0 && disp2 > 0.f) {
-      atomicMax(a.rep_d2 + r, __float_as_uint(disp2));
+      top2_insert(a.rep_d2 + 2 * r, disp2, 0.f);
     }
   } else {
     block_max_store(disp2, v2max, a.part_max);

@@ -1134,7 +1134,7 @@ int omdg_set_replicas(omdg_t h, int nrep, const int* off, const double* lo3, con
     dalloc(c.rep_e, 2 * (size_t)nrep);
     dalloc(c.rep_v2, nrep);
     dalloc(c.rep_vcur, nrep);
-    dalloc(c.rep_d2, nrep);
+    dalloc(c.rep_d2, 2 * (size_t)nrep);
     dalloc(c.rep_need, nrep);
     dalloc(c.qs, c.n);
     if (!c.dstep) dalloc(c.dstep, 1);
@@ -1147,7 +1147,7 @@ int omdg_set_replicas(omdg_t h, int nrep, const int* off, const double* lo3, con
     OMDG_HIP(hipMemset(c.dstep, 0, sizeof(int)));
     OMDG_HIP(hipMemset(c.rep_v2, 0, sizeof(unsigned long long) * nrep));
     OMDG_HIP(hipMemset(c.rep_vcur, 0, sizeof(double) * nrep));
-    OMDG_HIP(hipMemset(c.rep_d2, 0, sizeof(unsigned int) * nrep));
+    OMDG_HIP(hipMemset(c.rep_d2, 0, sizeof(unsigned int) * 2 * nrep));
     OMDG_HIP(hipMemset(c.rep_need, 0, sizeof(int) * nrep));
     c.rep_steps = 0;
     c.list_valid = false;

@@ -222,7 +222,7 @@ struct Ctx {
   int cap_scan = 0;
   int* cell_rep = nullptr;     // device: replica of each cell
   int* rep_need = nullptr;     // device: rebuild replica r in this step
-  unsigned int* rep_d2 = nullptr;  // device: max displacement^2 (float bits)
+  unsigned int* rep_d2 = nullptr;  // device: two largest displacements^2 per replica (float bits)
   double* rep_vcur = nullptr;  // device: max |v|^2 at the start of the step
   float rl2_type[kMaxTypes * kMaxTypes] = {0};  // list radius^2 per type pair
   // per-step modifies (omdg_set_modifies)
