@@ -4,7 +4,7 @@
 Default workload (--workload eam): fcc Al, a = 4.05 A, 40^3 cells = 256,000
 atoms, periodic, 300 K Maxwell-Boltzmann velocities plus 0.02 A Gaussian
 displacements (seeded), Johnson-type EAM tabulated to 5,000-point splines
-with cutoff 5.5 A, FP32 forces with compensated sums, FP64 velocity Verlet,
+with cutoff 5.5 A, FP32 pair terms summed per atom in FP64, FP64 velocity Verlet,
 dt = 2 fs.  One step = one velocity-Verlet step of all atoms (kick, drift,
 list check, density pass, force pass, kick).
 

@@ -2,10 +2,11 @@
  * omdg — C ABI of the MI355X (gfx950) force engine for Object-MD.
  *
  * One context owns the device copy of one system: FP64 positions and
- * velocities, forces (FP32 with compensated accumulation, or FP64), the
- * neighbour list and the potential tables.  All calls on a context must come
- * from one host thread; every call returns OMDG_OK (0) or an error code, and
- * omdg_last_error() gives the message of the last failure on that thread.
+ * velocities, forces (FP32 terms with FP64 or compensated per-atom sums, or
+ * FP64), the neighbour list and the potential tables.  All calls on a
+ * context must come from one host thread; every call returns OMDG_OK (0) or
+ * an error code, and omdg_last_error() gives the message of the last failure
+ * on that thread.
  *
  * Atom arrays crossing the boundary are in the caller's atom order (the
  * reference's AtomStorage order); the engine permutes atoms into cell order
@@ -85,7 +86,8 @@ int omdg_load_table(omdg_t ctx, int kind, int ta, int tb, const char* path,
  * TForceEAM::ComputeHalf (team.cpp:224-237); OMDG_POT_SW replaces
  * StillingerWeber (sw.cpp:118-273); OMDG_POT_TERSOFF replaces the Tersoff
  * bond order of tersoff.h:259-365.  fp64 = 1 computes forces in FP64
- * (plain method), 0 in FP32 with compensated per-atom sums. */
+ * (plain method), 0 in FP32 (per-atom sums in FP64 for EAM, compensated
+ * for the others). */
 int omdg_set_potential(omdg_t ctx, int potential, int fp64);
 
 /* Stillinger-Weber parameters in the order of tables/sw.silicon and
