@@ -169,6 +169,42 @@ def test_eam_shapes_vs_oracle(shape, maxnbr, monkeypatch):
     eng.close()
 
 
+@pytest.mark.parametrize("shape", ["default", "5"])
+def test_eam_rmult_tables_vs_oracle(shape, monkeypatch):
+    """PAIR and EDENS stored as r phi(r) and r f(r) (Format RMULT): both passes
+    take the value and derivative through the division by r of
+    treader.cpp:237-264, on the interleaved and the 16-lane path."""
+    if shape != "default":
+        monkeypatch.setenv("OMDG_EAM_SHAPE", shape)
+    s = eam_system(cells=10, seed=555)
+    t = eam_tables()
+    tab = {"EMBED": t["EMBED"]}
+    for k in ("PAIR", "EDENS"):
+        r = t[k].r0 + t[k].dr * np.arange(t[k].n)
+        tab[k] = tables.Table(r * t[k].y, t[k].dr, t[k].r0, rmult=True)
+    say(f"[eam-rmult] shape={shape} n={len(s['x'])}")
+    eng = om.Engine()
+    eng.set_box(s["lo"], s["hi"], s["pbc"])
+    eng.upload_atoms(s["x"], s["v"], mass=[s["mass"]])
+    for kind, k in ((om.TAB_PAIR, "PAIR"), (om.TAB_EDENS, "EDENS"), (om.TAB_EMBED, "EMBED")):
+        eng.add_table(kind, 0, 0, tab[k].y, tab[k].dr, tab[k].r0, tab[k].rmult)
+    eng.set_potential(om.POT_EAM)
+    eng.set_neighbor(0.3, 10, 0)
+    ep, vir = eng.compute()
+    f = eng.download(om.F)
+    rho = eng.download(om.RHO)
+    sp = [O.Spline.from_table(tab[k]) for k in ("PAIR", "EDENS", "EMBED")]
+    nb = O.neighbors(s["x"], s["lo"], s["L"], s["pbc"], 5.5 + 0.3, maxn=96)
+    fo, poto, viro, rhoo = O.eam_forces(s["x"], s["lo"], s["L"], s["pbc"], nb, *sp)
+    err = rel_rms(f, fo)
+    say(f"[eam-rmult] force relRMS={err:.3e} Ep={ep:.10f} oracle={poto.sum():.10f}")
+    assert err <= FORCE_RTOL
+    assert ep == pytest.approx(poto.sum(), rel=1e-6)
+    assert abs(vir - viro.sum() / 2) <= 2e-5 * np.abs(viro).sum()
+    assert rel_rms(rho, rhoo) < 1e-6
+    eng.close()
+
+
 def test_eam_perfect_crystal():
     pos, (lo, hi) = lattice.fcc100(8, 8, 8, 4.05)
     s = dict(x=pos, v=np.zeros_like(pos), lo=lo, hi=hi, L=hi - lo, pbc=7,
