@@ -87,7 +87,9 @@ int omdg_load_table(omdg_t ctx, int kind, int ta, int tb, const char* path,
  * StillingerWeber (sw.cpp:118-273); OMDG_POT_TERSOFF replaces the Tersoff
  * bond order of tersoff.h:259-365.  fp64 = 1 computes forces in FP64
  * (plain method), 0 in FP32 (per-atom sums in FP64 for EAM, compensated
- * for the others). */
+ * for the others).  EAM runs one species (the type-0 tables; the reference
+ * allows four, team.h:38): a force evaluation of a context that declares
+ * more atom types fails with OMDG_EINVAL. */
 int omdg_set_potential(omdg_t ctx, int potential, int fp64);
 
 /* Stillinger-Weber parameters in the order of tables/sw.silicon and
@@ -127,8 +129,10 @@ int omdg_surrogate_stats(omdg_t ctx, int* steps, int* fallback_steps,
  * verlet.radtole / verlet.n_mean (VerletList.cpp:46-52) and the refresh
  * cadence comm.refresh (systemgrid.cpp:565-617): the list radius is
  * cut + skin, displacements are checked every check_every steps and the
- * list is rebuilt before any atom moves more than skin/2.  maxnbr = 0
- * sizes the list from the first build. */
+ * list is rebuilt before any pair can have closed in by more than the skin
+ * (a single system before any atom moves skin/2; a replica when its two
+ * largest displacements sum to the skin).  maxnbr = 0 sizes the list from
+ * the first build. */
 int omdg_set_neighbor(omdg_t ctx, double skin, int check_every, int maxnbr);
 /* Build now from the current positions (VerletList::Update,
  * VerletList.cpp:126-199). */

@@ -228,6 +228,11 @@ static void check_ready(Ctx& c) {
   if (!c.box_set) throw ApiFail(OMDG_EINVAL, "box not set (omdg_set_box)");
   if (c.n <= 0) throw ApiFail(OMDG_EINVAL, "no atoms uploaded");
   if (c.potential == OMDG_POT_NONE) throw ApiFail(OMDG_EINVAL, "no potential selected");
+  // the EAM passes read the type-0 tables for every pair: one species only
+  // (the reference allows four, team.h:38)
+  if (c.potential == OMDG_POT_EAM && c.ntypes > 1)
+    throw ApiFail(OMDG_EINVAL, "EAM runs one species (atom types declared: " +
+                                   std::to_string(c.ntypes) + ")");
   upload_tables(c);
 }
 

@@ -330,6 +330,26 @@ def test_overlap_detected():
     eng.close()
 
 
+def test_eam_two_species_rejected():
+    """The EAM passes read the type-0 tables for every pair, so a system that
+    declares a second species is refused before any kernel runs."""
+    s = eam_system(cells=4)
+    eng = om.Engine()
+    eng.set_box(s["lo"], s["hi"], s["pbc"])
+    tid = np.zeros(len(s["x"]), np.int32)
+    tid[::2] = 1
+    eng.upload_atoms(s["x"], s["v"], tid=tid, mass=[s["mass"], s["mass"]])
+    t = eam_tables()
+    for kind, k in ((om.TAB_PAIR, "PAIR"), (om.TAB_EDENS, "EDENS"), (om.TAB_EMBED, "EMBED")):
+        eng.add_table(kind, 0, 0, t[k].y, t[k].dr, t[k].r0)
+    eng.set_potential(om.POT_EAM)
+    eng.set_neighbor(0.3, 10, 0)
+    with pytest.raises(om.OmdgError) as e:
+        eng.compute()
+    assert e.value.code == om.EINVAL
+    eng.close()
+
+
 def test_table_low_range_detected():
     s = lj_system(cells=4)
     x = s["x"].copy()
