@@ -145,7 +145,7 @@ __device__ inline void density_terms(const EamArgs& a, const float4* offs, const
     const float r2 = ex * ex + ey * ey + ez * ez;
     in[u] = ok[u] && (r2 <= a.cut_e2);
     rinv[u] = __builtin_amdgcn_rsqf(r2);  // inf for the pad's r2 = 0, masked below
-    seg[u] = spline_locate32_m(a.edens, in[u] ? r2 * rinv[u] : a.edens.r0, dx[u], qe);
+    seg[u] = spline_locate32_q(a.edens, in[u] ? r2 * rinv[u] : a.edens.r0, dx[u], qe);
   }
   float4 c[CHUNK];
 #pragma unroll
@@ -205,12 +205,12 @@ __device__ inline void force_terms(const EamArgs& a, const float4* offs, const f
     // masked below
     rinv[u] = __builtin_amdgcn_rsqf(r2[u]);
     const float r = in[u] ? r2[u] * rinv[u] : a.phi.r0;
-    sp[u] = spline_locate32_m(a.phi, r, dxp[u], qp);
+    sp[u] = spline_locate32_q(a.phi, r, dxp[u], qp);
     if (PACKED) {
       se[u] = sp[u];
       dxe[u] = dxp[u];
     } else {
-      se[u] = spline_locate32_m(a.edens, ine[u] ? r : a.edens.r0, dxe[u], qe);
+      se[u] = spline_locate32_q(a.edens, ine[u] ? r : a.edens.r0, dxe[u], qe);
     }
   }
   float4 cp[CHUNK], ce[CHUNK];

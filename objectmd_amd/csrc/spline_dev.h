@@ -52,19 +52,6 @@ __device__ inline bool spline_read32(const TabView& t, float r, float& val, floa
   return true;
 }
 
-__device__ inline bool spline_value32(const TabView& t, float r, float& val, int* err) {
-  int x;
-  float dx;
-  if (!spline_locate32(t, r, x, dx, err)) {
-    val = 0.f;
-    return false;
-  }
-  const float4 c = t.c32[x];
-  val = c.w + dx * (c.z + dx * (c.y + dx * c.x));
-  if (t.rmult) val /= r;
-  return true;
-}
-
 // FP64 segment location (as spline_locate32) with the error bits OR'd into
 // `err`, so a kernel can raise them with one atomic per lane at its end.
 __device__ inline bool spline_locate64_e(const TabView& t, double r, int& x, double& dx,
@@ -120,41 +107,10 @@ __device__ inline bool spline_read64(const TabView& t, double r, double& val, do
   return true;
 }
 
-__device__ inline bool spline_value64(const TabView& t, double r, double& val, int* err) {
-  int x;
-  double dx;
-  if (!spline_locate64(t, r, x, dx, err)) {
-    val = 0.0;
-    return false;
-  }
-  const double4 c = t.c64[x];
-  val = c.w + dx * (c.z + dx * (c.y + dx * c.x));
-  if (t.rmult) val /= r;
-  return true;
-}
-
-// Branch-free form of spline_locate32 for hot loops: range errors are OR'd
-// into `err` (one atomic per lane at the end of the kernel), and r is
-// clamped so the returned segment is always inside the table.
-__device__ inline int spline_locate32_nb(const TabView& t, float r, float& dx, int& err) {
-  const float rval = r - t.r0;
-  float qf = rval * t.inv_dr;
-  err |= !(qf < (float)t.n) ? ERR_RANGE_HI : 0;  // also NaN
-  err |= (qf <= -2.0f) ? ERR_RANGE_LO : 0;
-  qf = fminf(fmaxf(qf, -1.5f), (float)t.n - 0.5f);
-  int x = (int)qf;  // truncation, in [-1, n-1]
-  float d = rval - (float)x * t.dr;
-  const bool top = (x == t.n - 1);
-  d = top ? d + t.dr : d;
-  x = top ? x - 1 : x;
-  x = x < 0 ? 0 : x;
-  dx = d;
-  return x;
-}
-
 // Extremes of the scaled table coordinate (r - r0) / dr seen by one lane.
 // Range errors are raised once from them (range_flags) rather than per
-// lookup; the bounds are those of spline_locate32_nb.
+// lookup, with the bounds of spline_locate32: at or above n, or at or below
+// -2.
 struct QRange {
   float lo = 3.0e38f, hi = -3.0e38f;
   __device__ inline void add(float q) {
@@ -167,28 +123,13 @@ __device__ inline int range_flags(const TabView& t, const QRange& q) {
   return (q.hi >= (float)t.n ? ERR_RANGE_HI : 0) | (q.lo <= -2.0f ? ERR_RANGE_LO : 0);
 }
 
-// spline_locate32_nb with the range check deferred to range_flags; the
-// argument must not be NaN (callers pass r0 for pairs outside the cut).
+// Branch-free spline_locate32 for hot loops, the range check deferred to
+// range_flags (the argument must not be NaN: callers pass r0 for pairs
+// outside the cut).  q in [n - 1, n) is clamped to n - 1.5, giving x = n - 2
+// and dx = rval - (n - 2) dr: the last segment at dx + dr, as
+// spline_locate32 folds index n - 1; q in [-1.5, -1] keeps x = -1 for dx and
+// evaluates segment 0, as spline_locate32 does for index -1.
 __device__ inline int spline_locate32_q(const TabView& t, float r, float& dx, QRange& qr) {
-  const float rval = r - t.r0;
-  float qf = rval * t.inv_dr;
-  qr.add(qf);
-  qf = fminf(fmaxf(qf, -1.5f), (float)t.n - 0.5f);
-  int x = (int)qf;  // truncation, in [-1, n-1]
-  float d = rval - (float)x * t.dr;
-  const bool top = (x == t.n - 1);
-  d = top ? d + t.dr : d;
-  x = top ? x - 1 : x;
-  x = x < 0 ? 0 : x;
-  dx = d;
-  return x;
-}
-
-// spline_locate32_q with the top segment folded into the clamp: q in
-// [n - 1, n) is clamped to n - 1.5, giving x = n - 2 and dx = rval - (n - 2) dr,
-// the point spline_locate32_q reaches as x = n - 1 moved down one segment.
-// Three fewer operations per lookup; the range check is unchanged.
-__device__ inline int spline_locate32_m(const TabView& t, float r, float& dx, QRange& qr) {
   const float rval = r - t.r0;
   const float q = rval * t.inv_dr;
   qr.add(q);
