@@ -125,11 +125,12 @@ __device__ inline void eam_offsets(float4* offs, const EamArgs& a) {
   __syncthreads();
 }
 
-// Density terms of CHUNK list entries (ok: a live entry, not pad).
+// Density terms of CHUNK list entries (ok: a live entry, not pad); err
+// collects overlapping pairs (r^2 < 1e-6, gadget.cpp:149-159).
 template <int CHUNK>
 __device__ inline void density_terms(const EamArgs& a, const float4* offs, const float4& qi,
                                      const uint32_t (&e)[CHUNK], const bool (&ok)[CHUNK],
-                                     float& rho, QRange& qe) {
+                                     float& rho, QRange& qe, int& err) {
   float4 qj[CHUNK];
 #pragma unroll
   for (int u = 0; u < CHUNK; u++) qj[u] = ld_at(a.q, (e[u] & kSlotMask) << 4);
@@ -144,6 +145,7 @@ __device__ inline void density_terms(const EamArgs& a, const float4* offs, const
     const float ez = qi.z - qj[u].z - o.z;
     const float r2 = ex * ex + ey * ey + ez * ez;
     in[u] = ok[u] && (r2 <= a.cut_e2);
+    err |= (ok[u] && r2 < 1e-6f) ? ERR_OVERLAP : 0;
     rinv[u] = __builtin_amdgcn_rsqf(r2);  // inf for the pad's r2 = 0, masked below
     seg[u] = spline_locate32_q(a.edens, in[u] ? r2 * rinv[u] : a.edens.r0, dx[u], qe);
   }
@@ -302,6 +304,7 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
   const float4 qi = a.q[sl];
   const ListLanes<LANES> ll(a, sl, j, live ? a.nnbr[s] : 0);
   QRange qe;
+  int err = 0;
   float rho = 0.f;  // positive terms: a plain FP32 partial per lane
   for (int kk0 = 0; kk0 < ll.rem; kk0 += LANES * kPre) {
     uint32_t ew[kPre];
@@ -317,10 +320,10 @@ __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
         e[u] = ew[t0 + u];
         ok[u] = kk0 + (t0 + u) * LANES < ll.rem;
       }
-      density_terms<CHUNK>(a, offs, qi, e, ok, rho, qe);
+      density_terms<CHUNK>(a, offs, qi, e, ok, rho, qe, err);
     }
   }
-  density_store<ENERGY, LANES>(a, s, j, live, qi, rho, range_flags(a.edens, qe));
+  density_store<ENERGY, LANES>(a, s, j, live, qi, rho, err | range_flags(a.edens, qe));
 }
 
 template <bool ENERGY, bool PACKED, int LANES, int CHUNK>

@@ -330,6 +330,18 @@ def test_overlap_detected():
     eng.close()
 
 
+def test_eam_overlap_detected():
+    s = eam_system(cells=4)
+    x = s["x"].copy()
+    x[1] = x[0] + 1e-4
+    s["x"] = x
+    eng = _eam_engine(s)
+    with pytest.raises(om.OmdgError) as e:
+        eng.compute()
+    assert e.value.code == om.EOVERLAP
+    eng.close()
+
+
 def test_eam_two_species_rejected():
     """The EAM passes read the type-0 tables for every pair, so a system that
     declares a second species is refused before any kernel runs."""
