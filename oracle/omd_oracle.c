@@ -23,6 +23,12 @@
  *   orc_eam_forces         finite differences of Ep, sum f = 0, zero force in
  *                          the ideal crystal, virial = -sum r.f, analytic
  *                          Johnson functions through the spline.
+ *   orc_eam_alloy_forces   equal to orc_eam_forces when every species and
+ *                          cross table is the single-species one; finite
+ *                          differences of Ep with one density function and
+ *                          per-species embedding and pair tables; sum f = 0
+ *                          with distinct density tables (where the reference's
+ *                          own-species rho' makes f differ from -grad Ep).
  *   orc_sw_forces          ideal diamond at a=5.431 gives -2 eps per atom,
  *                          finite differences, sum f = 0, brute-force triplet
  *                          counts on the reference's small clusters.
@@ -631,6 +637,104 @@ int orc_eam_forces(int n, const double *xyz, const int *flag, const double *lo,
         if (!active(flag, i)) continue;
         double v;
         err = orc_spline_value(embed, myrho[i], &v);
+        pot[i] += v;
+    }
+    free(fp);
+    if (!rho) free(myrho);
+    return err;
+}
+
+/* Range of a table as TableReader::max_range (treader.h): r0 + (n-1) dr. */
+static double tab_range(const orc_table *t) { return t->r0 + (double)(t->n - 1) * t->dr; }
+
+/* EAM with several species (TEmbedding::AddTable, team.cpp:47-66;
+ * IterationNode :81-92; GetEmbDeriv/GetRhoDeriv :116-132; PostForce
+ * :135-145; TForceEAM::ComputeHalf :224-237).  Tables by atom type (nt
+ * types, n == 0 marks a missing table): edens[a*nt+b] the density table
+ * AtomID[a][b] (a == b the species' own EDENS, a != b the cross table, used
+ * for both orders), embed[a] the species' EMBED, phi[a*nt+b] the PAIR table
+ * of the TForceEAM registered for the type pair.  Density: rho_i and rho_j
+ * += edens[ti][tj](R) for R up to that table's range.  Force, for pairs with
+ * a PAIR table and R up to its range: fr = -(F'_i rho'_j + F'_j rho'_i +
+ * phi')/R, with rho'_x the derivative of atom x's own-species EDENS (zero
+ * beyond its range), as GetRhoDeriv takes it. */
+int orc_eam_alloy_forces(int n, const double *xyz, const int *type, const int *flag,
+                         const double *lo, const double *L, int pbc, const int *cnt,
+                         const int *code, int maxn, int nt, const orc_table *phi,
+                         const orc_table *edens, const orc_table *embed, double *f,
+                         double *pot, double *vir, double *rho) {
+    (void)lo;
+    (void)pbc;
+    zero_out(n, f, pot, vir);
+    double *myrho = rho ? rho : (double *)malloc(sizeof(double) * n);
+    double *fp = (double *)malloc(sizeof(double) * n);
+    memset(myrho, 0, sizeof(double) * n);
+    int err = 0;
+    for (int i = 0; i < n && !err; i++) {
+        if (!active(flag, i)) continue;
+        for (int k = 0; k < cnt[i]; k++) {
+            int c = code[(long)i * maxn + k], j, s[3];
+            decode(c, &j, s);
+            if (!half_owner(i, j, c % 27) || !active(flag, j)) continue;
+            const orc_table *te = &edens[type[i] * nt + type[j]];
+            if (te->n == 0) continue; /* no density table: not an EAM pair */
+            double d[3];
+            double r2 = pair_d(xyz, L, i, j, s, d);
+            if (r2 < ORC_PROXIMITY) {
+                err = 3;
+                break;
+            }
+            double R = sqrt(r2);
+            if (R <= tab_range(te)) {
+                double dens;
+                err = orc_spline_value(te, R, &dens);
+                if (err) break;
+                myrho[i] += dens;
+                myrho[j] += dens;
+            }
+        }
+    }
+    for (int i = 0; i < n && !err; i++) {
+        double v, dv;
+        if (!active(flag, i)) {
+            fp[i] = 0.0;
+            continue;
+        }
+        err = orc_spline_read(&embed[type[i]], myrho[i], &v, &dv);
+        fp[i] = dv;
+    }
+    for (int i = 0; i < n && !err; i++) {
+        if (!active(flag, i)) continue;
+        for (int k = 0; k < cnt[i]; k++) {
+            int c = code[(long)i * maxn + k], j, s[3];
+            decode(c, &j, s);
+            if (!half_owner(i, j, c % 27) || !active(flag, j)) continue;
+            const orc_table *tp = &phi[type[i] * nt + type[j]];
+            if (tp->n == 0) continue; /* no TForceEAM for this type pair */
+            double d[3];
+            double r2 = pair_d(xyz, L, i, j, s, d);
+            if (r2 < ORC_PROXIMITY) {
+                err = 3;
+                break;
+            }
+            double R = sqrt(r2);
+            if (R <= tab_range(tp)) {
+                double p, dp, v, drho_i = 0.0, drho_j = 0.0;
+                err = orc_spline_read(tp, R, &p, &dp);
+                if (err) break;
+                const orc_table *oi = &edens[type[i] * nt + type[i]];
+                const orc_table *oj = &edens[type[j] * nt + type[j]];
+                if (R <= tab_range(oi) && (err = orc_spline_read(oi, R, &v, &drho_i))) break;
+                if (R <= tab_range(oj) && (err = orc_spline_read(oj, R, &v, &drho_j))) break;
+                double fr = -(fp[i] * drho_j + fp[j] * drho_i + dp) / R;
+                return_force(i, j, d, fr, p, f, pot, vir);
+            }
+        }
+    }
+    for (int i = 0; i < n && !err && pot; i++) {
+        if (!active(flag, i)) continue;
+        double v;
+        err = orc_spline_value(&embed[type[i]], myrho[i], &v);
         pot[i] += v;
     }
     free(fp);

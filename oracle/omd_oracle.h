@@ -65,6 +65,13 @@ int orc_eam_forces(int n, const double *xyz, const int *flag, const double *lo,
                    int maxn, const orc_table *phi, const orc_table *edens,
                    const orc_table *embed, double *f, double *pot, double *vir,
                    double *rho);
+/* Several species: tables by type, phi and edens nt*nt, embed nt; n == 0
+ * marks a missing table. */
+int orc_eam_alloy_forces(int n, const double *xyz, const int *type, const int *flag,
+                         const double *lo, const double *L, int pbc, const int *cnt,
+                         const int *code, int maxn, int nt, const orc_table *phi,
+                         const orc_table *edens, const orc_table *embed, double *f,
+                         double *pot, double *vir, double *rho);
 
 /* Stillinger-Weber parameter block (tables/sw.silicon keys, sw.cpp:35-93). */
 typedef struct {

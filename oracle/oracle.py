@@ -186,6 +186,40 @@ def eam_forces(x, lo, L, pbc, nbr, phi, edens, embed, flag=None):
     return f, pot, vir, rho
 
 
+def eam_alloy_forces(x, lo, L, pbc, nbr, types, nt, phi, edens, embed, flag=None):
+    """EAM with nt species: phi and edens map (a, b) -> Spline (absent pairs
+    have no table; edens[(a, b)] is used for both orders), embed maps a ->
+    Spline."""
+    x, lo, L = _geom(x, lo, L)
+    cnt, code = nbr
+    n = len(x)
+    f = np.zeros((n, 3))
+    pot = np.zeros(n)
+    vir = np.zeros(n)
+    rho = np.zeros(n)
+    fl = None if flag is None else np.ascontiguousarray(flag, np.int32)
+    ty = np.ascontiguousarray(types, np.int32)
+    empty = Table(0, 0, 0.0, 0.0, None)
+
+    def grid(tabs, sym):
+        arr = (Table * (nt * nt))()
+        for a in range(nt):
+            for b in range(nt):
+                sp = tabs.get((a, b)) or (tabs.get((b, a)) if sym else None)
+                arr[a * nt + b] = sp.t if sp is not None else empty
+        return arr
+
+    ph = grid(phi, True)
+    ed = grid(edens, True)
+    em = (Table * nt)(*[embed[a].t if a in embed else empty for a in range(nt)])
+    rc = lib().orc_eam_alloy_forces(n, _d(x), _i(ty), _i(fl), _d(lo), _d(L), pbc, _i(cnt),
+                                    _i(code), code.shape[1], nt, ph, ed, em, _d(f), _d(pot),
+                                    _d(vir), _d(rho))
+    if rc:
+        raise RuntimeError(f"orc_eam_alloy_forces = {rc}")
+    return f, pot, vir, rho
+
+
 def sw_forces(x, lo, L, pbc, nbr, p, flag=None):
     x, lo, L = _geom(x, lo, L)
     cnt, code = nbr

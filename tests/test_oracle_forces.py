@@ -192,3 +192,76 @@ def test_tersoff_finite_difference(R, D, disp):
 
     for i, d, fd in _fd_check(energy, x, [0, 5, 33, 60], h=1e-6):
         assert f[i, d] == pytest.approx(fd, rel=1e-5, abs=1e-7)
+
+
+# ---------------------------------------------------------------------------
+# EAM with two species (TEmbedding::AddTable per species and type pair)
+# ---------------------------------------------------------------------------
+def _alloy_tables(phi_scale, rho_scale, emb_scale):
+    """Johnson tables scaled per type pair / species: phi_scale and rho_scale
+    map (a, b) -> factor, emb_scale maps a -> factor."""
+    t = eam_tables()
+
+    def sc(tab, c):
+        return O.Spline.from_table(tables.Table(c * tab.y, tab.dr, tab.r0))
+    phi = {k: sc(t["PAIR"], c) for k, c in phi_scale.items()}
+    rho = {k: sc(t["EDENS"], c) for k, c in rho_scale.items()}
+    emb = {k: sc(t["EMBED"], c) for k, c in emb_scale.items()}
+    return phi, rho, emb
+
+
+def _alloy_system(seed=3):
+    s = eam_system(cells=3, disp=0.08, seed=seed)
+    types = np.random.default_rng(seed).integers(0, 2, len(s["x"])).astype(np.int32)
+    return s, types
+
+
+def test_eam_alloy_reduces_to_single_species():
+    s, types = _alloy_system()
+    nb = O.neighbors(s["x"], s["lo"], s["L"], 7, 5.5 + 0.5)
+    one = {(0, 0): 1.0, (0, 1): 1.0, (1, 1): 1.0}
+    phi, rho, emb = _alloy_tables(one, one, {0: 1.0, 1: 1.0})
+    fa, pa, va, ra = O.eam_alloy_forces(s["x"], s["lo"], s["L"], 7, nb, types, 2, phi, rho, emb)
+    f1, p1, v1, r1 = O.eam_forces(s["x"], s["lo"], s["L"], 7, nb, *_eam_splines())
+    assert np.array_equal(ra, r1)
+    assert np.abs(fa - f1).max() < 1e-12 * np.abs(f1).max()
+    assert np.abs(pa - p1).max() < 1e-12
+    assert np.abs(va - v1).max() < 1e-12 * np.abs(v1).max()
+
+
+def test_eam_alloy_finite_difference():
+    """One density function for every pair (so the reference's own-species
+    rho' is the true derivative) with per-pair PAIR and per-species EMBED
+    tables: f = -grad Ep and sum f = 0."""
+    s, types = _alloy_system(seed=4)
+    nb = O.neighbors(s["x"], s["lo"], s["L"], 7, 5.5 + 0.5)
+    phi, rho, emb = _alloy_tables({(0, 0): 1.0, (0, 1): 1.1, (1, 1): 1.25},
+                                  {(0, 0): 1.0, (0, 1): 1.0, (1, 1): 1.0}, {0: 1.0, 1: 0.8})
+
+    def energy(x):
+        return O.eam_alloy_forces(x, s["lo"], s["L"], 7, nb, types, 2, phi, rho, emb)[1].sum()
+
+    f = O.eam_alloy_forces(s["x"], s["lo"], s["L"], 7, nb, types, 2, phi, rho, emb)[0]
+    assert np.abs(f.sum(0)).max() < 1e-10
+    atoms = [int(np.flatnonzero(types == 0)[0]), int(np.flatnonzero(types == 1)[0]), 55]
+    for i, d, fd in _fd_check(energy, s["x"], atoms):
+        assert f[i, d] == pytest.approx(fd, rel=2e-5, abs=2e-7)
+
+
+def test_eam_alloy_distinct_densities_third_law():
+    """Distinct own and cross density tables: the pair force stays central
+    (sum f = 0), and a species' density counts its cross pairs with the
+    cross table (rho of an atom with only cross neighbours scales with it)."""
+    s, types = _alloy_system(seed=5)
+    nb = O.neighbors(s["x"], s["lo"], s["L"], 7, 5.5 + 0.5)
+    phi, rho, emb = _alloy_tables({(0, 0): 1.0, (0, 1): 1.0, (1, 1): 1.0},
+                                  {(0, 0): 1.0, (0, 1): 0.85, (1, 1): 0.7}, {0: 1.0, 1: 1.0})
+    f, pot, vir, r = O.eam_alloy_forces(s["x"], s["lo"], s["L"], 7, nb, types, 2, phi, rho, emb)
+    assert np.abs(f.sum(0)).max() < 1e-10
+    # all-A and all-B versions of the same positions bracket the densities
+    rA = O.eam_alloy_forces(s["x"], s["lo"], s["L"], 7, nb, np.zeros_like(types), 2, phi, rho,
+                            emb)[3]
+    rB = O.eam_alloy_forces(s["x"], s["lo"], s["L"], 7, nb, np.ones_like(types), 2, phi, rho,
+                            emb)[3]
+    assert np.allclose(rB, 0.7 * rA, rtol=1e-12)
+    assert np.all(r <= rA * (1 + 1e-12)) and np.all(r >= 0.7 * rA * (1 - 1e-12))
