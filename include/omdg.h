@@ -87,9 +87,10 @@ int omdg_load_table(omdg_t ctx, int kind, int ta, int tb, const char* path,
  * StillingerWeber (sw.cpp:118-273); OMDG_POT_TERSOFF replaces the Tersoff
  * bond order of tersoff.h:259-365.  fp64 = 1 computes forces in FP64
  * (plain method), 0 in FP32 (per-atom sums in FP64 for EAM, compensated
- * for the others).  EAM runs one species (the type-0 tables; the reference
- * allows four, team.h:38): a force evaluation of a context that declares
- * more atom types fails with OMDG_EINVAL. */
+ * for the others).  EAM with several species (up to four, team.h:38) takes
+ * a PAIR table per type pair that interacts, an EDENS table per type pair
+ * (own and cross) and an EMBED table per species; a species without its own
+ * EDENS and EMBED tables fails with OMDG_EINVAL. */
 int omdg_set_potential(omdg_t ctx, int potential, int fp64);
 
 /* Stillinger-Weber parameters in the order of tables/sw.silicon and

@@ -105,10 +105,10 @@ struct ListLanes {
   uint32_t lane_off;  // bytes: ((j >> 2) n + s) * 16 + (j & 3) * 4
   int rem;            // entries of this lane: kk < rem
   uint32_t pad;
-  __device__ ListLanes(const EamArgs& a, int sl, int j, int nn) {
-    nbr = a.nbr;
-    row_words = (size_t)a.n * 4;
-    lane_off = ((uint32_t)(j >> 2) * (uint32_t)a.n + (uint32_t)sl) * 16u + (uint32_t)(j & 3) * 4u;
+  __device__ ListLanes(const uint32_t* list, int n, int sl, int j, int nn) {
+    nbr = list;
+    row_words = (size_t)n * 4;
+    lane_off = ((uint32_t)(j >> 2) * (uint32_t)n + (uint32_t)sl) * 16u + (uint32_t)(j & 3) * 4u;
     rem = nn - j;
     pad = (uint32_t)sl | (31u << kImgShift);
   }
@@ -119,8 +119,8 @@ struct ListLanes {
 };
 
 // Per-block table of the stencil offsets; codes 27..31 (the pad's 31) read zero.
-__device__ inline void eam_offsets(float4* offs, const EamArgs& a) {
-  load_offsets(offs, a.hx, a.hy, a.hz);
+__device__ inline void eam_offsets(float4* offs, float hx, float hy, float hz) {
+  load_offsets(offs, hx, hy, hz);
   if (threadIdx.x >= 27 && threadIdx.x < 32) offs[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
   __syncthreads();
 }
@@ -296,13 +296,13 @@ template <bool ENERGY, int LANES, int CHUNK>
 __global__ void __launch_bounds__(kBlock) k_eam_density(EamArgs a) {
   constexpr int kPre = pre_words<LANES, CHUNK>();
   __shared__ float4 offs[32];
-  eam_offsets(offs, a);
+  eam_offsets(offs, a.hx, a.hy, a.hz);
   const int g = xcd_block() * blockDim.x + threadIdx.x;
   const int s = g / LANES, j = g % LANES;
   const bool live = s < a.n;  // no early exit: all lanes take part in the shuffles
   const int sl = live ? s : 0;  // a valid slot for the unconditional loads
   const float4 qi = a.q[sl];
-  const ListLanes<LANES> ll(a, sl, j, live ? a.nnbr[s] : 0);
+  const ListLanes<LANES> ll(a.nbr, a.n, sl, j, live ? a.nnbr[s] : 0);
   QRange qe;
   int err = 0;
   float rho = 0.f;  // positive terms: a plain FP32 partial per lane
@@ -330,13 +330,13 @@ template <bool ENERGY, bool PACKED, int LANES, int CHUNK>
 __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
   constexpr int kPre = pre_words<LANES, CHUNK>();
   __shared__ float4 offs[32];
-  eam_offsets(offs, a);
+  eam_offsets(offs, a.hx, a.hy, a.hz);
   const int g = xcd_block() * blockDim.x + threadIdx.x;
   const int s = g / LANES, j = g % LANES;
   const bool live = s < a.n;
   const int sl = live ? s : 0;
   const float4 qi = a.qf[sl];
-  const ListLanes<LANES> ll(a, sl, j, live ? a.nnbr[s] : 0);
+  const ListLanes<LANES> ll(a.nbr, a.n, sl, j, live ? a.nnbr[s] : 0);
   QRange qp, qe;
   ForceAcc acc;
   for (int kk0 = 0; kk0 < ll.rem; kk0 += LANES * kPre) {
@@ -357,6 +357,165 @@ __global__ void __launch_bounds__(kBlock) k_eam_force(EamArgs a) {
     }
   }
   force_store<ENERGY, PACKED, LANES>(a, s, j, live, acc, qp, qe);
+}
+
+// ---- several species -------------------------------------------------------
+// TEmbedding with a density table per type pair (AtomID[a][b]) and an
+// embedding function per species (team.cpp:47-145), TForceEAM per type pair
+// (team.cpp:224-237): rho_i += dens[ti][tj](r); F'(rho) of the atom's
+// species; fr = -(F'_i rho'_j + F'_j rho'_i + phi'_ij) / r with rho' from
+// each atom's own-species table, zero beyond its range (GetRhoDeriv,
+// team.cpp:128-132); pairs without a PAIR table get no force.  Four lanes
+// per atom, one list entry at a time, FP32 spline reads.
+struct EamAlloyArgs {
+  const float4* q;  // snapshots: the w word carries the type
+  const uint32_t* nbr;
+  const int* nnbr;
+  float* rho;
+  float4* qf;
+  float* emb;
+  float* f;
+  float* pot;
+  float* vir;
+  int* err;
+  int n;
+  float hx, hy, hz;
+  TabView phi[kMaxTypes * kMaxTypes];   // n == 0: no force for the type pair
+  TabView dens[kMaxTypes * kMaxTypes];  // n == 0: no density for the type pair
+  TabView embed[kMaxTypes];
+  float phi_cut2[kMaxTypes * kMaxTypes], dens_cut2[kMaxTypes * kMaxTypes];
+  float own_cut[kMaxTypes];  // range of each species' own density table
+};
+
+constexpr int kAlloyLanes = 4;
+
+__global__ void __launch_bounds__(kBlock) k_eam_alloy_density(EamAlloyArgs a, int energy) {
+  __shared__ float4 offs[32];
+  eam_offsets(offs, a.hx, a.hy, a.hz);
+  const int g = xcd_block() * blockDim.x + threadIdx.x;
+  const int s = g / kAlloyLanes, j = g % kAlloyLanes;
+  const bool live = s < a.n;
+  const int sl = live ? s : 0;
+  const float4 qi = a.q[sl];
+  const int ti = snap_type(qi.w);
+  const ListLanes<kAlloyLanes> ll(a.nbr, a.n, sl, j, live ? a.nnbr[s] : 0);
+  float rho = 0.f;
+  int err = 0;
+  for (int kk = 0; kk < ll.rem; kk += kAlloyLanes) {
+    const uint32_t e = ll.word(kk);
+    const float4 qj = ld_at(a.q, (e & kSlotMask) << 4);
+    const int slot = ti * kMaxTypes + snap_type(qj.w);
+    const float4 o = offs[e >> kImgShift];
+    const float dx = qi.x - qj.x - o.x, dy = qi.y - qj.y - o.y, dz = qi.z - qj.z - o.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 < 1e-6f) {
+      err |= ERR_OVERLAP;
+      continue;
+    }
+    if (a.dens[slot].n == 0 || r2 > a.dens_cut2[slot]) continue;
+    float v, dv;
+    if (spline_read32(a.dens[slot], sqrtf(r2), v, dv, a.err)) rho += v;
+  }
+  if (err) atomicOr(a.err, err);
+  const double rs = lanes_sum<kAlloyLanes>((double)rho);
+  if (live && j == 0) {
+    a.rho[s] = (float)rs;
+    double F, dF;
+    spline_read64(a.embed[ti], (double)(float)rs, F, dF, a.err);
+    a.qf[s] = make_float4(qi.x, qi.y, qi.z, (float)dF);
+    if (energy) a.emb[s] = (float)F;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_eam_alloy_force(EamAlloyArgs a, int energy) {
+  __shared__ float4 offs[32];
+  eam_offsets(offs, a.hx, a.hy, a.hz);
+  const int g = xcd_block() * blockDim.x + threadIdx.x;
+  const int s = g / kAlloyLanes, j = g % kAlloyLanes;
+  const bool live = s < a.n;
+  const int sl = live ? s : 0;
+  const float4 qi = a.qf[sl];
+  const int ti = snap_type(a.q[sl].w);
+  const ListLanes<kAlloyLanes> ll(a.nbr, a.n, sl, j, live ? a.nnbr[s] : 0);
+  double fx = 0.0, fy = 0.0, fz = 0.0, pot = 0.0, vir = 0.0;
+  for (int kk = 0; kk < ll.rem; kk += kAlloyLanes) {
+    const uint32_t e = ll.word(kk);
+    const uint32_t off = (e & kSlotMask) << 4;
+    const float4 qj = ld_at(a.qf, off);
+    const int tj = snap_type(ld_at(reinterpret_cast<const float*>(a.q), off + 12u));
+    const int slot = ti * kMaxTypes + tj;
+    if (a.phi[slot].n == 0) continue;
+    const float4 o = offs[e >> kImgShift];
+    const float dx = qi.x - qj.x - o.x, dy = qi.y - qj.y - o.y, dz = qi.z - qj.z - o.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 > a.phi_cut2[slot]) continue;
+    const float r = sqrtf(r2);
+    float p, dp, v, drho_i = 0.f, drho_j = 0.f;
+    if (!spline_read32(a.phi[slot], r, p, dp, a.err)) continue;
+    if (r <= a.own_cut[ti]) spline_read32(a.dens[ti * (kMaxTypes + 1)], r, v, drho_i, a.err);
+    if (r <= a.own_cut[tj]) spline_read32(a.dens[tj * (kMaxTypes + 1)], r, v, drho_j, a.err);
+    const float fr = -(qi.w * drho_j + qj.w * drho_i + dp) / r;
+    fx += (double)(fr * dx);
+    fy += (double)(fr * dy);
+    fz += (double)(fr * dz);
+    if (energy) {
+      pot += (double)(0.5f * p);
+      vir += (double)(fr * r2);
+    }
+  }
+  const double sx = lanes_sum<kAlloyLanes>(fx), sy = lanes_sum<kAlloyLanes>(fy),
+               sz = lanes_sum<kAlloyLanes>(fz);
+  const double sp = lanes_sum<kAlloyLanes>(pot), sv = lanes_sum<kAlloyLanes>(vir);
+  if (live && j == 0) {
+    const int n = a.n;
+    a.f[s] = (float)sx;
+    a.f[(size_t)n + s] = (float)sy;
+    a.f[2 * (size_t)n + s] = (float)sz;
+    if (energy) {
+      a.pot[s] = (float)(sp + (double)a.emb[s]);
+      a.vir[s] = (float)sv;
+    }
+  }
+}
+
+void eam_alloy_forces(Ctx& c, bool energy) {
+  EamAlloyArgs a;
+  a.q = c.q;
+  a.nbr = c.nbr;
+  a.nnbr = c.nnbr;
+  a.rho = c.rho;
+  a.qf = c.qf;
+  a.emb = c.emb;
+  a.f = c.f32;
+  a.pot = c.pot32;
+  a.vir = c.vir32;
+  a.err = c.dstat + ST_ERR;
+  a.n = c.n;
+  a.hx = (float)c.grid.h[0];
+  a.hy = (float)c.grid.h[1];
+  a.hz = (float)c.grid.h[2];
+  for (int ta = 0; ta < kMaxTypes; ta++) {
+    for (int tb = 0; tb < kMaxTypes; tb++) {
+      const int k = ta * kMaxTypes + tb;
+      const bool in = ta < c.ntypes && tb < c.ntypes;
+      a.phi[k] = tab_view(c, in ? c.tab_index[OMDG_TAB_PAIR][ta][tb] : -1);
+      a.dens[k] = tab_view(c, in ? c.tab_index[OMDG_TAB_EDENS][ta][tb] : -1);
+      a.phi_cut2[k] = a.phi[k].n ? (float)(a.phi[k].rcut * a.phi[k].rcut) : 0.f;
+      a.dens_cut2[k] = a.dens[k].n ? (float)(a.dens[k].rcut * a.dens[k].rcut) : 0.f;
+    }
+    a.embed[ta] = tab_view(c, ta < c.ntypes ? c.tab_index[OMDG_TAB_EMBED][ta][ta] : -1);
+    a.own_cut[ta] = (float)a.dens[ta * (kMaxTypes + 1)].rcut;
+  }
+  const int nb = grid_blocks((long)c.n * kAlloyLanes, kBlock);
+  {
+    TimerScope ts(c, TG_PASS1);
+    k_eam_alloy_density<<<nb, kBlock, 0, c.stream>>>(a, energy ? 1 : 0);
+  }
+  {
+    TimerScope ts(c, TG_FORCE);
+    k_eam_alloy_force<<<nb, kBlock, 0, c.stream>>>(a, energy ? 1 : 0);
+  }
+  OMDG_HIP(hipGetLastError());
 }
 
 // Interleave the PAIR and EDENS coefficients when they share one grid.
@@ -458,6 +617,10 @@ int tune_pass(Ctx& c, const EamArgs& a, bool pass1) {
 }  // namespace
 
 void eam_forces(Ctx& c, bool energy) {
+  if (c.ntypes > 1) {
+    eam_alloy_forces(c, energy);
+    return;
+  }
   EamArgs a;
   a.q = c.q;
   a.nbr = c.nbr;

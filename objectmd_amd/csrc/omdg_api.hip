@@ -228,11 +228,13 @@ static void check_ready(Ctx& c) {
   if (!c.box_set) throw ApiFail(OMDG_EINVAL, "box not set (omdg_set_box)");
   if (c.n <= 0) throw ApiFail(OMDG_EINVAL, "no atoms uploaded");
   if (c.potential == OMDG_POT_NONE) throw ApiFail(OMDG_EINVAL, "no potential selected");
-  // the EAM passes read the type-0 tables for every pair: one species only
-  // (the reference allows four, team.h:38)
-  if (c.potential == OMDG_POT_EAM && c.ntypes > 1)
-    throw ApiFail(OMDG_EINVAL, "EAM runs one species (atom types declared: " +
-                                   std::to_string(c.ntypes) + ")");
+  // every EAM species needs its own density and embedding tables
+  // (TEmbedding::AddTable, team.cpp:47-53)
+  if (c.potential == OMDG_POT_EAM)
+    for (int t = 0; t < c.ntypes; t++)
+      if (c.tab_index[OMDG_TAB_EDENS][t][t] < 0 || c.tab_index[OMDG_TAB_EMBED][t][t] < 0)
+        throw ApiFail(OMDG_EINVAL, "EAM species " + std::to_string(t) +
+                                       " needs its EDENS and EMBED tables");
   upload_tables(c);
 }
 

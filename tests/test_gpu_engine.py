@@ -371,20 +371,62 @@ def test_eam_overlap_detected():
     eng.close()
 
 
-def test_eam_two_species_rejected():
-    """The EAM passes read the type-0 tables for every pair, so a system that
-    declares a second species is refused before any kernel runs."""
-    s = eam_system(cells=4)
+def _alloy_setup(s, types, phi_sc, rho_sc, emb_sc, skip_own_b=False):
+    """Engine and oracle tables for two species from scaled Johnson tables."""
+    t = eam_tables()
     eng = om.Engine()
     eng.set_box(s["lo"], s["hi"], s["pbc"])
-    tid = np.zeros(len(s["x"]), np.int32)
-    tid[::2] = 1
-    eng.upload_atoms(s["x"], s["v"], tid=tid, mass=[s["mass"], s["mass"]])
-    t = eam_tables()
-    for kind, k in ((om.TAB_PAIR, "PAIR"), (om.TAB_EDENS, "EDENS"), (om.TAB_EMBED, "EMBED")):
-        eng.add_table(kind, 0, 0, t[k].y, t[k].dr, t[k].r0)
+    eng.upload_atoms(s["x"], s["v"], tid=types, mass=[s["mass"], 0.9 * s["mass"]])
+    phi, rho, emb = {}, {}, {}
+    for (ta, tb), c in phi_sc.items():
+        eng.add_table(om.TAB_PAIR, ta, tb, c * t["PAIR"].y, t["PAIR"].dr, t["PAIR"].r0)
+        phi[(ta, tb)] = O.Spline(c * t["PAIR"].y, t["PAIR"].dr, t["PAIR"].r0)
+    for (ta, tb), c in rho_sc.items():
+        if skip_own_b and ta == tb == 1:
+            continue
+        eng.add_table(om.TAB_EDENS, ta, tb, c * t["EDENS"].y, t["EDENS"].dr, t["EDENS"].r0)
+        rho[(ta, tb)] = O.Spline(c * t["EDENS"].y, t["EDENS"].dr, t["EDENS"].r0)
+    for ta, c in emb_sc.items():
+        eng.add_table(om.TAB_EMBED, ta, ta, c * t["EMBED"].y, t["EMBED"].dr, t["EMBED"].r0)
+        emb[ta] = O.Spline(c * t["EMBED"].y, t["EMBED"].dr, t["EMBED"].r0)
     eng.set_potential(om.POT_EAM)
     eng.set_neighbor(0.3, 10, 0)
+    return eng, phi, rho, emb
+
+
+def test_eam_alloy_vs_oracle():
+    """Two species with their own and a cross density table, three pair
+    tables and two embedding functions, against the oracle's
+    TEmbedding/TForceEAM with several species."""
+    s = eam_system(cells=10, seed=777)
+    types = np.random.default_rng(9).integers(0, 2, len(s["x"])).astype(np.int32)
+    eng, phi, rho, emb = _alloy_setup(s, types, {(0, 0): 1.0, (0, 1): 1.1, (1, 1): 1.25},
+                                      {(0, 0): 1.0, (0, 1): 0.85, (1, 1): 0.7},
+                                      {0: 1.0, 1: 0.8})
+    ep, vir = eng.compute()
+    f = eng.download(om.F)
+    r = eng.download(om.RHO)
+    nb = O.neighbors(s["x"], s["lo"], s["L"], s["pbc"], 5.5 + 0.3, maxn=96)
+    fo, poto, viro, rhoo = O.eam_alloy_forces(s["x"], s["lo"], s["L"], s["pbc"], nb, types, 2,
+                                              phi, rho, emb)
+    err = rel_rms(f, fo)
+    say(f"[eam-alloy] force relRMS={err:.3e} Ep={ep:.10f} oracle={poto.sum():.10f}")
+    assert err <= FORCE_RTOL
+    assert ep == pytest.approx(poto.sum(), rel=1e-6)
+    assert abs(vir - viro.sum() / 2) <= 2e-5 * np.abs(viro).sum()
+    assert rel_rms(r, rhoo) < 1e-6
+    eng.close()
+
+
+def test_eam_species_without_tables_rejected():
+    """A declared species without its own EDENS table is refused before any
+    kernel runs (TEmbedding needs AtomID[t][t] for every species)."""
+    s = eam_system(cells=4)
+    types = np.zeros(len(s["x"]), np.int32)
+    types[::2] = 1
+    eng, _, _, _ = _alloy_setup(s, types, {(0, 0): 1.0, (0, 1): 1.0, (1, 1): 1.0},
+                                {(0, 0): 1.0, (0, 1): 1.0, (1, 1): 1.0}, {0: 1.0, 1: 1.0},
+                                skip_own_b=True)
     with pytest.raises(om.OmdgError) as e:
         eng.compute()
     assert e.value.code == om.EINVAL
