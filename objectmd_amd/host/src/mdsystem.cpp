@@ -18,13 +18,16 @@
 namespace omd {
 
 // ------------------------------------------------------------ lowering ----
+// TForceEAM::Init + TEmbedding::AddTable (team.cpp:47-66, 206-222): the
+// PAIR table of the type pair; for a species with itself its own EDENS and
+// EMBED, for two species the cross EDENS, all from eam.<name>.
 int TForceEAM::Lower(omdg_t eng) {
   const string path = search_path("$OMD_TABLE", "eam." + tablefile);
-  if (AtomTypeA != 0 || AtomTypeB != 0)
-    die("the device EAM kernel handles one species (type 0)");
-  check_rc(omdg_load_table(eng, OMDG_TAB_PAIR, 0, 0, path.c_str(), "PAIR"), "eam PAIR");
-  check_rc(omdg_load_table(eng, OMDG_TAB_EDENS, 0, 0, path.c_str(), "EDENS"), "eam EDENS");
-  check_rc(omdg_load_table(eng, OMDG_TAB_EMBED, 0, 0, path.c_str(), "EMBED"), "eam EMBED");
+  const int a = AtomTypeA, b = AtomTypeB;
+  check_rc(omdg_load_table(eng, OMDG_TAB_PAIR, a, b, path.c_str(), "PAIR"), "eam PAIR");
+  check_rc(omdg_load_table(eng, OMDG_TAB_EDENS, a, b, path.c_str(), "EDENS"), "eam EDENS");
+  if (a == b)
+    check_rc(omdg_load_table(eng, OMDG_TAB_EMBED, a, a, path.c_str(), "EMBED"), "eam EMBED");
   return OMDG_POT_EAM;
 }
 
