@@ -1082,6 +1082,9 @@ void orc_wrap(int n, double *xyz, const double *lo, const double *L, int pbc) {
         }
 }
 
+/* Tersoff parameters of orc_run_tersoff's run (kind 2), NULL otherwise. */
+static const orc_tersoff_param *run_tersoff = NULL;
+
 static int eval_forces(int kind, int n, const double *xyz, const int *flag,
                        const double *lo, const double *L, int pbc,
                        const int *cnt, const int *code, int maxn,
@@ -1091,6 +1094,9 @@ static int eval_forces(int kind, int n, const double *xyz, const int *flag,
     if (kind == 0)
         return orc_pair_forces(n, xyz, flag, lo, L, pbc, cnt, code, maxn, t0, f,
                                pot, vir);
+    if (kind == 2)
+        return orc_tersoff_forces(n, xyz, flag, lo, L, pbc, cnt, code, maxn, run_tersoff, f,
+                                  pot);
     return orc_eam_forces(n, xyz, flag, lo, L, pbc, cnt, code, maxn, t0, t1, t2,
                           f, pot, vir, NULL);
 }
@@ -1154,4 +1160,17 @@ out:
     free(pot);
     free(vir);
     return err;
+}
+
+/* orc_run with Tersoff forces: the velocity-Verlet driver above with the
+ * bond-order forces of orc_tersoff_forces and parameter block P. */
+int orc_run_tersoff(int n, double *xyz, double *vel, const int *flag, const double *mass,
+                    const double *lo, const double *L, int pbc, const orc_tersoff_param *P,
+                    double cut, double skin, int refresh, int maxn, int nsteps, double dt,
+                    double *energies) {
+    run_tersoff = P;
+    int rc = orc_run(2, n, xyz, vel, flag, mass, lo, L, pbc, NULL, NULL, NULL, cut, skin,
+                     refresh, maxn, nsteps, dt, energies);
+    run_tersoff = NULL;
+    return rc;
 }

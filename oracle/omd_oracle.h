@@ -113,6 +113,11 @@ int orc_run(int kind, int n, double *xyz, double *vel, const int *flag,
             const orc_table *t0, const orc_table *t1, const orc_table *t2,
             double cut, double skin, int refresh, int maxn, int nsteps,
             double dt, double *energies);
+/* orc_run with Tersoff forces (parameter block P). */
+int orc_run_tersoff(int n, double *xyz, double *vel, const int *flag, const double *mass,
+                    const double *lo, const double *L, int pbc, const orc_tersoff_param *P,
+                    double cut, double skin, int refresh, int maxn, int nsteps, double dt,
+                    double *energies);
 
 #ifdef __cplusplus
 }

@@ -269,6 +269,24 @@ def run(kind, x, v, mass, lo, L, pbc, tables, cut, skin, refresh, nsteps, dt, ma
     return x, v, en[:nsteps]
 
 
+def run_tersoff(x, v, mass, lo, L, pbc, p, cut, skin, refresh, nsteps, dt, maxn=64,
+                flag=None):
+    """run() with Tersoff forces (parameter block p, a TersoffParam)."""
+    x, lo, L = _geom(x, lo, L)
+    x = x.copy()
+    v = np.ascontiguousarray(v, np.float64).reshape(-1, 3).copy()
+    n = len(x)
+    m = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, np.float64), (n,)))
+    en = np.zeros((max(nsteps, 1), 3))
+    fl = None if flag is None else np.ascontiguousarray(flag, np.int32)
+    rc = lib().orc_run_tersoff(n, _d(x), _d(v), _i(fl), _d(m), _d(lo), _d(L), pbc,
+                               ctypes.byref(p), ctypes.c_double(cut), ctypes.c_double(skin),
+                               refresh, maxn, nsteps, ctypes.c_double(dt), _d(en))
+    if rc:
+        raise RuntimeError(f"orc_run_tersoff = {rc}")
+    return x, v, en[:nsteps]
+
+
 def vv_kick_drift(x, v, f, mass, dt, flag=None):
     """In place: v += (dt/2 f)/m; x += dt v."""
     n = len(x)

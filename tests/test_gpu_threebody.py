@@ -169,3 +169,40 @@ def test_tersoff_steps_conserve_energy():
     say(f"[tersoff-drift] E/N range {np.ptp(e):.3e}")
     assert np.ptp(e) < 2e-4
     eng.close()
+
+
+@pytest.mark.skip(reason="open: the FP32-force Tersoff run drifts -5.1e-4 eV/atom per 1e4 "
+                  "steps on this slab against the FP64 oracle's -5.4e-5 (range 1.0e-3 "
+                  "vs 6.2e-4); the bound below is the target")
+def test_tersoff_drift_1e4_steps_vs_oracle():
+    """Total-energy drift over 10^4 steps (512-atom Si(001) slab at 1,000 K,
+    dt 1 fs): the FP32-force GPU run drifts no more than the FP64 oracle's
+    velocity Verlet with the same Tersoff forces."""
+    x, lo, hi = _si(4, 0.05, 3, 3, a=5.432)
+    m = tables.mass_sim(tables.MASS_SI)
+    v = lattice.maxwell_boltzmann(len(x), m, 1000.0, 4)
+    n = len(x)
+    eng = om.Engine()
+    eng.set_box(lo, hi, 3)
+    eng.upload_atoms(x, v, mass=[m])
+    eng.set_tersoff(tersoff_params(2.7, 0.15))
+    eng.set_potential(om.POT_TERSOFF)
+    eng.set_neighbor(0.5, 5, 0)
+    base, _ = eng.compute()
+    eng.zero_forces()
+    eg = []
+    for _ in range(100):
+        ek, ep = eng.step(100, 0.001)
+        eg.append((ek + ep - base) / n)
+    eng.close()
+    eg = np.array(eg)
+    _, _, en = O.run_tersoff(x, v, m, lo, hi - lo, 3, _ters_oracle(2.7, 0.15),
+                             2.85, 0.5, 5, 10000, 0.001)
+    eo = en[99::100, 2] / n
+    steps = np.arange(1, 101) * 100.0
+    slope_g = np.polyfit(steps, eg, 1)[0] * 1e4
+    slope_o = np.polyfit(steps, eo, 1)[0] * 1e4
+    say(f"[tersoff-drift-1e4] slope GPU {slope_g:.3e} oracle {slope_o:.3e} eV/atom per 1e4 "
+        f"steps; range GPU {np.ptp(eg):.3e} oracle {np.ptp(eo):.3e}")
+    assert abs(slope_g) <= max(1.5 * abs(slope_o), 2e-6)
+    assert np.ptp(eg) <= 1.5 * np.ptp(eo) + 1e-6
